@@ -1,0 +1,48 @@
+// kernels.hpp - launchers shared between the translation units of libcmag_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+
+#include "cmag_params.h"
+
+namespace cmag {
+
+int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
+int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, uint32_t* index,
+                     uint32_t n, hipStream_t s);
+int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,
+                   const uint32_t* hash, const uint32_t* index, const float* oldPos,
+                   const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s);
+
+// stable LSD radix sort.  Sorts (keys, vals) using (alt_keys, alt_vals) as the
+// ping-pong buffers; *result_in_alt tells where the sorted pair ended up.
+size_t radix_sort_workspace_bytes(uint32_t n);
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
+                     uint32_t n, int key_bits, void* workspace, size_t workspace_bytes,
+                     bool* result_in_alt, hipStream_t s);
+
+int launch_density(const cmag_params& P, float* measures, const float* sortedPos,
+                   const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, hipStream_t s);
+int launch_force(const cmag_params& P, float* acc, const float* measures, const float* sortedPos,
+                 const float* sortedVel, const uint32_t* index, const uint32_t* cellStart,
+                 const uint32_t* cellEnd, uint32_t n, hipStream_t s);
+int launch_pressure_force(const cmag_params& P, float* presF, const float* measures,
+                          const float* sortedPos, const uint32_t* index, const uint32_t* cellStart,
+                          const uint32_t* cellEnd, uint32_t n, hipStream_t s);
+int launch_viscous_force(const cmag_params& P, float* viscF, const float* measures,
+                         const float* sortedPos, const float* sortedVel, const uint32_t* index,
+                         const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                         hipStream_t s);
+int launch_integrate(const cmag_params& P, float* pos, float* vel, float* velLF, const float* acc,
+                     const float* presF, float wave_time, uint32_t n, hipStream_t s);
+int launch_configure_boundary(const cmag_params& P, float* pos, float wave_height, uint32_t n,
+                              hipStream_t s);
+int launch_shift_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
+int launch_remove_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
+
+}  // namespace cmag

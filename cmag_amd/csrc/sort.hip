@@ -1,0 +1,327 @@
+// sort.hip - stable LSD radix sort_by_key for uint32 keys/values (gfx950).
+//
+// Replaces thrust::sort_by_key of sortParticles (DamBreak.Core/fluidSystem.cu:94-99).
+// Stability is what makes the reference's permutation reproducible: index
+// starts as the identity every step, so the expected result is exactly the
+// stable argsort of the hash.
+//
+// One pass per 8-bit digit, three launches per pass (reduce-then-scan):
+//   rs_hist      per-tile digit histogram, written digit-major so that
+//   scan         one exclusive scan gives every (digit, tile) its global base
+//   rs_scatter   stable in-tile ranking with wave64 ballots (8 ballots give
+//                the mask of lanes holding the same digit), a local sort of
+//                the tile in LDS, then coalesced runs written per digit.
+// No inter-workgroup waiting anywhere, so no launch order assumptions.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace cmag {
+
+constexpr int RS_NT = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_NT * RS_ITEMS;  // 4096 keys per workgroup
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// exclusive prefix of v over the workgroup; lds needs NT/64 words
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+    constexpr int NW = NT / 64;
+    const int lane = __lane_id(), w = threadIdx.x / 64;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        uint32_t x = lds[k];
+        pre += (k < w) ? x : 0u;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// lanes of this wave whose 8-bit digit equals mine (among valid lanes)
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        uint32_t bit = (d >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// ---- exclusive scan of a uint32 array -------------------------------------
+__global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const uint32_t* __restrict__ in,
+                                                               size_t len,
+                                                               uint32_t* __restrict__ partial) {
+    __shared__ uint32_t lds[SCAN_NT / 64];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + (size_t)k * SCAN_NT + threadIdx.x;
+        sum += (i < len) ? in[i] : 0u;
+    }
+    uint32_t tot;
+    block_exclusive_scan<SCAN_NT>(sum, lds, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// each thread owns SCAN_ITEMS consecutive elements; `carry` (may be null)
+// holds the exclusive base of every block
+__global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const uint32_t* __restrict__ in,
+                                                             uint32_t* __restrict__ out, size_t len,
+                                                             const uint32_t* __restrict__ carry) {
+    __shared__ uint32_t lds[SCAN_NT / 64];
+    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + k;
+        v[k] = (i < len) ? in[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan<SCAN_NT>(sum, lds, &tot);
+    if (carry) run += carry[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        size_t i = base + k;
+        if (i < len) out[i] = run;
+        run += v[k];
+    }
+}
+
+static size_t scan_levels_words(size_t len) {
+    size_t words = 0;
+    while (len > (size_t)SCAN_TILE) {
+        len = (len + SCAN_TILE - 1) / SCAN_TILE;
+        words += len;
+    }
+    return words;
+}
+
+// out = exclusive_scan(in); workspace holds scan_levels_words(len) words
+static int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
+                              hipStream_t s) {
+    if (len == 0) return CMAG_OK;
+    if (len <= (size_t)SCAN_TILE) {
+        hipLaunchKernelGGL(scan_down_kernel, dim3(1), dim3(SCAN_NT), 0, s, in, out, len,
+                           (const uint32_t*)nullptr);
+        return check_launch("scan_down_kernel");
+    }
+    size_t blocks = (len + SCAN_TILE - 1) / SCAN_TILE;
+    uint32_t* partial = ws;
+    hipLaunchKernelGGL(scan_reduce_kernel, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, in, len,
+                       partial);
+    CMAG_TRY(check_launch("scan_reduce_kernel"));
+    // scan the partials in place (recursively for very long inputs)
+    CMAG_TRY(exclusive_scan_u32(partial, partial, blocks, ws + blocks, s));
+    hipLaunchKernelGGL(scan_down_kernel, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, in, out, len,
+                       (const uint32_t*)partial);
+    return check_launch("scan_down_kernel");
+}
+
+// ---- radix passes ----------------------------------------------------------
+__global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
+                                                         int shift, uint32_t* __restrict__ counts,
+                                                         uint32_t numTiles) {
+    __shared__ uint32_t hist[256];
+    const int tid = threadIdx.x, lane = __lane_id();
+    hist[tid] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (uint32_t)RS_TILE;
+    uint32_t d[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+        uint32_t idx = base + k * RS_NT + tid;
+        d[k] = idx < n ? (keys[idx] >> shift) & 255u : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+        uint32_t idx = base + k * RS_NT + tid;
+        bool valid = idx < n;
+        uint64_t m = match_digit(d[k], valid);
+        if (valid && (__ffsll((unsigned long long)m) - 1) == lane)
+            atomicAdd(&hist[d[k]], (uint32_t)__popcll(m));
+    }
+    __syncthreads();
+    counts[(size_t)tid * numTiles + blockIdx.x] = hist[tid];
+}
+
+__global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
+    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets, uint32_t numTiles) {
+    __shared__ uint32_t s_keys[RS_TILE];
+    __shared__ uint32_t s_vals[RS_TILE];
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_run[256];
+    __shared__ uint32_t s_wcnt[RS_NT / 64][256];
+    __shared__ uint32_t s_scan[RS_NT / 64];
+
+    const int tid = threadIdx.x, lane = __lane_id(), w = tid / 64;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t base = tile * (uint32_t)RS_TILE;
+    const uint32_t cnt = min((uint32_t)RS_TILE, n - base);
+
+    uint32_t rk[RS_ITEMS], rv[RS_ITEMS];
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+        uint32_t idx = base + k * RS_NT + tid;
+        rk[k] = idx < n ? keys_in[idx] : 0u;
+        rv[k] = idx < n ? vals_in[idx] : 0u;
+    }
+
+    uint32_t c = counts[(size_t)tid * numTiles + tile];
+    s_base[tid] = offsets[(size_t)tid * numTiles + tile];
+    uint32_t tot;
+    s_start[tid] = block_exclusive_scan<RS_NT>(c, s_scan, &tot);
+    s_run[tid] = 0;
+
+    const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+#pragma unroll
+        for (int ww = 0; ww < RS_NT / 64; ++ww) s_wcnt[ww][tid] = 0;
+        __syncthreads();
+        uint32_t idx = base + k * RS_NT + tid;
+        bool valid = idx < n;
+        uint32_t dg = (rk[k] >> shift) & 255u;
+        uint64_t m = match_digit(dg, valid);
+        uint32_t rank = (uint32_t)__popcll(m & lt_mask);
+        if (valid && (__ffsll((unsigned long long)m) - 1) == lane) s_wcnt[w][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            uint32_t run = s_run[tid];
+#pragma unroll
+            for (int ww = 0; ww < RS_NT / 64; ++ww) {
+                uint32_t x = s_wcnt[ww][tid];
+                s_wcnt[ww][tid] = run;
+                run += x;
+            }
+            s_run[tid] = run;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t p = s_start[dg] + s_wcnt[w][dg] + rank;
+            s_keys[p] = rk[k];
+            s_vals[p] = rv[k];
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) {
+        uint32_t p = k * RS_NT + tid;
+        if (p < cnt) {
+            uint32_t key = s_keys[p];
+            uint32_t dg = (key >> shift) & 255u;
+            uint32_t dst = s_base[dg] + (p - s_start[dg]);
+            keys_out[dst] = key;
+            vals_out[dst] = s_vals[p];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_u32_kernel(uint32_t* __restrict__ dst,
+                                                        const uint32_t* __restrict__ src, size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s) {
+    if (n == 0) return CMAG_OK;
+    uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(copy_u32_kernel, dim3(blocks), dim3(256), 0, s, dst, src, n);
+    return check_launch("copy_u32_kernel");
+}
+
+static uint32_t num_tiles(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+size_t radix_sort_workspace_bytes(uint32_t n) {
+    size_t len = (size_t)256 * num_tiles(n);
+    size_t words = 2 * len + scan_levels_words(len);
+    return (words * sizeof(uint32_t) + 255) & ~(size_t)255;
+}
+
+int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_t* alt_vals,
+                     uint32_t n, int key_bits, void* workspace, size_t workspace_bytes,
+                     bool* result_in_alt, hipStream_t s) {
+    *result_in_alt = false;
+    if (key_bits < 0 || key_bits > 32) return fail(CMAG_E_INVALID, "key_bits outside [0,32]");
+    if (n < 2 || key_bits == 0) return CMAG_OK;
+    if (workspace_bytes < radix_sort_workspace_bytes(n))
+        return fail(CMAG_E_SIZE, "radix sort workspace too small");
+    const uint32_t tiles = num_tiles(n);
+    const size_t len = (size_t)256 * tiles;
+    uint32_t* counts = static_cast<uint32_t*>(workspace);
+    uint32_t* offsets = counts + len;
+    uint32_t* scan_ws = offsets + len;
+    uint32_t *kin = keys, *vin = vals, *kout = alt_keys, *vout = alt_vals;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(rs_hist_kernel, dim3(tiles), dim3(RS_NT), 0, s, kin, n, shift, counts, tiles);
+        CMAG_TRY(check_launch("rs_hist_kernel"));
+        CMAG_TRY(exclusive_scan_u32(counts, offsets, len, scan_ws, s));
+        hipLaunchKernelGGL(rs_scatter_kernel, dim3(tiles), dim3(RS_NT), 0, s, kin, vin, kout, vout, n,
+                           shift, counts, offsets, tiles);
+        CMAG_TRY(check_launch("rs_scatter_kernel"));
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        *result_in_alt = !*result_in_alt;
+    }
+    return CMAG_OK;
+}
+
+}  // namespace cmag
+
+using namespace cmag;
+
+extern "C" {
+
+size_t cmag_sort_scratch_bytes(uint32_t n, int key_bits) {
+    (void)key_bits;
+    size_t pairs = ((size_t)n * 2 * sizeof(uint32_t) + 255) & ~(size_t)255;
+    return pairs + radix_sort_workspace_bytes(n);
+}
+
+int cmag_sort_by_key(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, void* scratch,
+                     size_t scratch_bytes, cmag_stream_t stream) {
+    if (n && (!keys || !vals)) return fail(CMAG_E_INVALID, "cmag_sort_by_key: null pointer");
+    if (n < 2 || key_bits == 0) return CMAG_OK;
+    if (!scratch || scratch_bytes < cmag_sort_scratch_bytes(n, key_bits))
+        return fail(CMAG_E_SIZE, "cmag_sort_by_key: scratch smaller than cmag_sort_scratch_bytes");
+    hipStream_t s = as_stream(stream);
+    uint32_t* alt_keys = static_cast<uint32_t*>(scratch);
+    uint32_t* alt_vals = alt_keys + n;
+    size_t pairs = ((size_t)n * 2 * sizeof(uint32_t) + 255) & ~(size_t)255;
+    void* ws = static_cast<char*>(scratch) + pairs;
+    bool in_alt = false;
+    CMAG_TRY(radix_sort_pairs(keys, vals, alt_keys, alt_vals, n, key_bits, ws,
+                              scratch_bytes - pairs, &in_alt, s));
+    if (in_alt) {
+        CMAG_TRY(launch_copy_u32(keys, alt_keys, n, s));
+        CMAG_TRY(launch_copy_u32(vals, alt_vals, n, s));
+    }
+    return CMAG_OK;
+}
+
+}  // extern "C"

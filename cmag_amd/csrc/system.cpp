@@ -1,0 +1,483 @@
+// system.cpp - System objects behind the C ABI: device buffers, reset(),
+// update() in the reference stage order, setArray/getArray, HIP-graph replay.
+//
+// DamBreakSystem       DamBreak.Core/fluidSystem.cpp
+// PoiseuilleFlowSystem Poiseuille.Core/poiseuilleFlowSystem.cpp
+// PeristalsisSystem    Peristalsis.Core/PeristalsisSystem.cpp
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace cmag {
+int launch_peristalsis_forces(const cmag_params& P, float* viscF, float* presF, const float* measures,
+                              const float* sortedPos, const float* sortedVel, const uint32_t* index,
+                              const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                              hipStream_t s);
+}
+
+using namespace cmag;
+
+enum RefKind { REF_NONE = -1 };
+
+struct cmag_system {
+    cmag_params P{};
+    int ref_kind = REF_NONE;  // system kind when built by a reference constructor
+    uint32_t n = 0, numCells = 0;
+    int key_bits = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+
+    float *pos = nullptr, *vel = nullptr, *velLF = nullptr, *acc = nullptr, *presF = nullptr;
+    float *measures = nullptr, *sortedPos = nullptr, *sortedVel = nullptr, *predicted = nullptr;
+    uint32_t *hash = nullptr, *index = nullptr, *hashAlt = nullptr, *indexAlt = nullptr;
+    uint32_t *cellStart = nullptr, *cellEnd = nullptr;
+    void* sortWs = nullptr;
+    size_t sortWsBytes = 0;
+
+    float elapsed = 0.0f;
+    // Peristalsis boundary-configuration state, PeristalsisSystem.cpp:330-343
+    float time_shift = 0.0f, time_relax = 0.0f, wave_height = 0.0f;
+
+    hipGraphExec_t graph = nullptr;
+    bool graph_valid = false;
+
+    ~cmag_system() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
+                        hash, index, hashAlt, indexAlt, cellStart, cellEnd, sortWs};
+        for (void* b : bufs)
+            if (b) (void)hipFree(b);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+
+    int alloc() {
+        const size_t f4 = sizeof(float) * 4 * (size_t)(n ? n : 1);
+        const size_t u = sizeof(uint32_t) * (size_t)(n ? n : 1);
+        CMAG_HIP_TRY(hipStreamCreate(&own_stream));
+        stream = own_stream;
+        float** fb[] = {&pos, &vel, &velLF, &acc, &measures, &sortedPos, &sortedVel};
+        for (float** b : fb) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
+        if (P.system == CMAG_SYS_PERISTALSIS) {
+            CMAG_HIP_TRY(hipMalloc((void**)&presF, f4));
+            CMAG_HIP_TRY(hipMalloc((void**)&predicted, f4));
+        }
+        uint32_t** ub[] = {&hash, &index, &hashAlt, &indexAlt};
+        for (uint32_t** b : ub) CMAG_HIP_TRY(hipMalloc((void**)b, u));
+        CMAG_HIP_TRY(hipMalloc((void**)&cellStart, sizeof(uint32_t) * (size_t)numCells));
+        CMAG_HIP_TRY(hipMalloc((void**)&cellEnd, sizeof(uint32_t) * (size_t)numCells));
+        sortWsBytes = radix_sort_workspace_bytes(n);
+        CMAG_HIP_TRY(hipMalloc(&sortWs, sortWsBytes));
+        float* zb[] = {pos, vel, velLF, acc, measures, sortedPos, sortedVel, presF, predicted};
+        for (float* b : zb)
+            if (b) CMAG_HIP_TRY(hipMemsetAsync(b, 0, f4, stream));
+        for (uint32_t* b : {hash, index, hashAlt, indexAlt}) CMAG_HIP_TRY(hipMemsetAsync(b, 0, u, stream));
+        CMAG_HIP_TRY(hipMemsetAsync(cellStart, 0xff, sizeof(uint32_t) * (size_t)numCells, stream));
+        CMAG_HIP_TRY(hipMemsetAsync(cellEnd, 0, sizeof(uint32_t) * (size_t)numCells, stream));
+        CMAG_HIP_TRY(hipStreamSynchronize(stream));
+        return CMAG_OK;
+    }
+
+    int passes() const { return (key_bits + 7) / 8; }
+
+    // one update(): hash -> sort -> reorder/cellStart -> density -> forces -> integrate
+    int enqueue_step(float wave_time) {
+        // the hash goes to whichever buffer makes the sort end in (hash, index)
+        const bool odd = passes() % 2 == 1;
+        uint32_t* hk = odd ? hashAlt : hash;
+        uint32_t* hv = odd ? indexAlt : index;
+        CMAG_TRY(launch_calc_hash(P, pos, hk, hv, n, stream));
+        bool in_alt = false;
+        CMAG_TRY(radix_sort_pairs(hk, hv, odd ? hash : hashAlt, odd ? index : indexAlt, n, key_bits,
+                                  sortWs, sortWsBytes, &in_alt, stream));
+        const bool sorted_moved = n >= 2 && key_bits > 0;
+        if (sorted_moved && in_alt != odd)
+            return fail(CMAG_E_STATE, "radix sort ended in an unexpected buffer");
+        if (!sorted_moved && odd) {
+            // no pass ran: move the keys from where the hash put them
+            CMAG_TRY(launch_copy_u32(hash, hk, n, stream));
+            CMAG_TRY(launch_copy_u32(index, hv, n, stream));
+        }
+        CMAG_TRY(launch_reorder(cellStart, cellEnd, sortedPos, sortedVel, hash, index, pos, velLF, n,
+                                numCells, stream));
+        CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream));
+        if (P.system == CMAG_SYS_PERISTALSIS) {
+            CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
+                                               cellStart, cellEnd, n, stream));
+        } else {
+            CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
+                                  stream));
+        }
+        return launch_integrate(P, pos, vel, velLF, acc, presF, wave_time, n, stream);
+    }
+
+    // PeristalsisSystem::Update's host state machine, PeristalsisSystem.cpp:330-343;
+    // returns the wave time t_w of the step
+    int peristalsis_prologue(float* t_w) {
+        *t_w = 0.0f;
+        if (P.system != CMAG_SYS_PERISTALSIS) return CMAG_OK;
+        if (P.is_boundary_configuration) {
+            time_shift += P.delta_time;
+            if (wave_height < P.amplitude) {
+                CMAG_TRY(launch_configure_boundary(P, pos, wave_height, n, stream));
+                wave_height += P.delta_time * powf(10.0f, -3.0f);
+            } else {
+                time_relax -= P.delta_time;
+                if (time_relax < 0) P.is_boundary_configuration = !P.is_boundary_configuration;
+            }
+        }
+        *t_w = P.is_boundary_configuration ? 0.0f : elapsed - time_shift;
+        return CMAG_OK;
+    }
+
+    int update() {
+        float t_w = 0.0f;
+        CMAG_TRY(peristalsis_prologue(&t_w));
+        CMAG_TRY(enqueue_step(t_w));
+        elapsed += P.delta_time;
+        return CMAG_OK;
+    }
+
+    void invalidate_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_valid = false;
+    }
+
+    int build_graph() {
+        invalidate_graph();
+        if (stream == nullptr) return fail(CMAG_E_STATE, "graph capture needs a non-default stream");
+        hipGraph_t g = nullptr;
+        CMAG_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step(0.0f);
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc != CMAG_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        graph_valid = true;
+        return CMAG_OK;
+    }
+
+    int run(uint32_t steps, bool use_graph) {
+        if (!use_graph || P.system == CMAG_SYS_PERISTALSIS) {
+            for (uint32_t k = 0; k < steps; ++k) CMAG_TRY(update());
+            return CMAG_OK;
+        }
+        if (!graph_valid) CMAG_TRY(build_graph());
+        for (uint32_t k = 0; k < steps; ++k) {
+            CMAG_HIP_TRY(hipGraphLaunch(graph, stream));
+            elapsed += P.delta_time;
+        }
+        return CMAG_OK;
+    }
+
+    float* farray(int a) {
+        switch (a) {
+        case CMAG_ARRAY_POSITION: return pos;
+        case CMAG_ARRAY_VELOCITY: return vel;
+        case CMAG_ARRAY_MEASURES: return measures;
+        case CMAG_ARRAY_ACCELERATION: return acc;
+        case CMAG_ARRAY_VELOCITYLEAPFROG: return velLF;
+        case CMAG_ARRAY_PRESSUREFORCE: return presF;
+        case CMAG_ARRAY_SORTEDPOS: return sortedPos;
+        case CMAG_ARRAY_SORTEDVEL: return sortedVel;
+        case CMAG_ARRAY_PREDICTEDPOSITION: return predicted;
+        default: return nullptr;
+        }
+    }
+    uint32_t* uarray(int a, uint32_t* len) {
+        switch (a) {
+        case CMAG_ARRAY_HASH: *len = n; return hash;
+        case CMAG_ARRAY_INDEX: *len = n; return index;
+        case CMAG_ARRAY_CELLSTART: *len = numCells; return cellStart;
+        case CMAG_ARRAY_CELLEND: *len = numCells; return cellEnd;
+        default: *len = 0; return nullptr;
+        }
+    }
+
+    int set_array(int a, const float* data, int start, int count) {
+        float* d = farray(a);
+        if (!d) return fail(CMAG_E_INVALID, "setArray: unknown array for this system");
+        if (start < 0 || count < 0 || (uint64_t)start + (uint64_t)count > n)
+            return fail(CMAG_E_INVALID, "setArray: range outside the particle arrays");
+        if (count == 0) return CMAG_OK;
+        if (!data) return fail(CMAG_E_INVALID, "setArray: null data");
+        CMAG_HIP_TRY(hipMemcpyAsync(d + 4 * (size_t)start, data, sizeof(float) * 4 * (size_t)count,
+                                    hipMemcpyHostToDevice, stream));
+        CMAG_HIP_TRY(hipStreamSynchronize(stream));
+        return CMAG_OK;
+    }
+
+    int get_array(int a, float* data, int start, int count) {
+        float* d = farray(a);
+        if (!d) return fail(CMAG_E_INVALID, "getArray: unknown array for this system");
+        if (start < 0 || count < 0 || (uint64_t)start + (uint64_t)count > n)
+            return fail(CMAG_E_INVALID, "getArray: range outside the particle arrays");
+        if (count == 0) return CMAG_OK;
+        if (!data) return fail(CMAG_E_INVALID, "getArray: null data");
+        CMAG_HIP_TRY(hipMemcpyAsync(data, d + 4 * (size_t)start, sizeof(float) * 4 * (size_t)count,
+                                    hipMemcpyDeviceToHost, stream));
+        CMAG_HIP_TRY(hipStreamSynchronize(stream));
+        return CMAG_OK;
+    }
+};
+
+namespace cmag {
+// reference reset() initial states (shared with nothing else; the oracle has
+// its own restatement for the tests)
+void init_dambreak(const cmag_params& P, float* pos4, uint32_t n);
+void init_poiseuille(const cmag_params& P, float* pos4, uint32_t n);
+void init_peristalsis(const cmag_params& P, float* pos4, uint32_t n);
+float peristalsis_lattice_sum(const cmag_params& P, const float* pos4, uint32_t n);
+}  // namespace cmag
+
+static int make_system(const cmag_params& P, uint32_t n, int ref_kind, cmag_system** out) {
+    if (!out) return fail(CMAG_E_INVALID, "null output handle");
+    *out = nullptr;
+    CMAG_TRY(validate_params(&P));
+    cmag_system* s = new (std::nothrow) cmag_system();
+    if (!s) return fail(CMAG_E_INVALID, "out of host memory");
+    s->P = P;
+    s->ref_kind = ref_kind;
+    s->n = n;
+    s->numCells = cmag_params_num_cells(&P);
+    s->key_bits = cmag_params_key_bits(&P);
+    s->time_relax = 1000 * P.delta_time;
+    int rc = s->alloc();
+    if (rc != CMAG_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return CMAG_OK;
+}
+
+extern "C" {
+
+int cmag_dambreak_create(uint32_t fx, uint32_t fy, uint32_t fz, int32_t bo, uint32_t gx, uint32_t gy,
+                         uint32_t gz, float r, cmag_system** out) {
+    cmag_params P;
+    CMAG_TRY(cmag_params_dambreak(&P, fx, fy, fz, bo, gx, gy, gz, r));
+    if (bo < 1) return fail(CMAG_E_INVALID, "DamBreak needs boundaryOffset >= 1");
+    // fluidSystem.cpp:34-37
+    uint32_t n = fx * fy * fz + gx * (uint32_t)bo + 2u * (gy - (uint32_t)bo) * (uint32_t)bo;
+    return make_system(P, n, CMAG_SYS_DAMBREAK, out);
+}
+
+int cmag_poiseuille_create(uint32_t fx, uint32_t fy, uint32_t fz, int32_t bo, uint32_t gx, uint32_t gy,
+                           uint32_t gz, float r, cmag_system** out) {
+    cmag_params P;
+    CMAG_TRY(cmag_params_poiseuille(&P, fx, fy, fz, bo, gx, gy, gz, r));
+    // poiseuilleFlowSystem.cpp:36-37
+    uint32_t n = fx * fy * fz + 2u * gx * (uint32_t)bo;
+    return make_system(P, n, CMAG_SYS_POISEUILLE, out);
+}
+
+int cmag_peristalsis_create(float dt, uint32_t fx, uint32_t fy, uint32_t fz, float amplitude,
+                            float wave_speed, float soundspeed, float gravx, float gravy, float gravz,
+                            int32_t bo, uint32_t gx, uint32_t gy, uint32_t gz, float radius,
+                            cmag_system** out) {
+    cmag_params P;
+    CMAG_TRY(cmag_params_peristalsis(&P, dt, fx, fy, fz, amplitude, wave_speed, soundspeed, gravx, gravy,
+                                     gravz, bo, gx, gy, gz, radius));
+    // PeristalsisSystem.cpp:36-37
+    uint32_t n = fx * fy * fz + 2u * gx * (uint32_t)bo;
+    return make_system(P, n, CMAG_SYS_PERISTALSIS, out);
+}
+
+int cmag_system_create(const cmag_params* p, uint32_t n, cmag_system** out) {
+    if (!p) return fail(CMAG_E_INVALID, "null parameter block");
+    return make_system(*p, n, REF_NONE, out);
+}
+
+int cmag_reference_initial_state(const cmag_params* p, uint32_t n, float* pos4) {
+    CMAG_TRY(validate_params(p));
+    if (n && !pos4) return fail(CMAG_E_INVALID, "null output");
+    if (p->system == CMAG_SYS_DAMBREAK) init_dambreak(*p, pos4, n);
+    else if (p->system == CMAG_SYS_POISEUILLE) init_poiseuille(*p, pos4, n);
+    else init_peristalsis(*p, pos4, n);
+    return CMAG_OK;
+}
+
+float cmag_peristalsis_lattice_sum(const cmag_params* p, const float* pos4, uint32_t n) {
+    if (!p || !pos4 || n == 0) return 0.0f;
+    return peristalsis_lattice_sum(*p, pos4, n);
+}
+
+int cmag_system_destroy(cmag_system* s) {
+    if (s) {
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        delete s;
+    }
+    return CMAG_OK;
+}
+
+int cmag_system_reset(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (s->ref_kind == REF_NONE)
+        return fail(CMAG_E_STATE, "reset() exists only for systems built by a reference constructor");
+    s->invalidate_graph();
+    const uint32_t n = s->n;
+    std::vector<float> hpos(4 * (size_t)n, 0.0f), zero(4 * (size_t)n, 0.0f), meas(4 * (size_t)n, 0.0f);
+    s->elapsed = 0.0f;
+    if (s->ref_kind == CMAG_SYS_PERISTALSIS) {
+        // PeristalsisSystem::Reset, PeristalsisSystem.cpp:213-237
+        s->time_shift = 0.0f;
+        s->time_relax = 1000 * s->P.delta_time;
+        s->wave_height = 0.0f;
+        s->P.is_boundary_configuration = 1;
+        init_peristalsis(s->P, hpos.data(), n);
+        CMAG_TRY(s->set_array(CMAG_ARRAY_POSITION, hpos.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITY, zero.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_MEASURES, zero.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_ACCELERATION, zero.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_PRESSUREFORCE, zero.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITYLEAPFROG, zero.data(), 0, (int)n));
+        CMAG_TRY(s->set_array(CMAG_ARRAY_PREDICTEDPOSITION, zero.data(), 0, (int)n));
+        s->P.particle_mass = s->P.rest_density / peristalsis_lattice_sum(s->P, hpos.data(), n);
+        return CMAG_OK;
+    }
+    // DamBreakSystem::reset fluidSystem.cpp:361-380; PoiseuilleFlowSystem::reset
+    // poiseuilleFlowSystem.cpp:311-323.  hMeasures starts as (restDensity, 0, 0, 0).
+    if (s->ref_kind == CMAG_SYS_DAMBREAK) init_dambreak(s->P, hpos.data(), n);
+    else init_poiseuille(s->P, hpos.data(), n);
+    for (uint32_t i = 0; i < n; ++i) meas[4 * (size_t)i] = s->P.rest_density;
+    CMAG_TRY(s->set_array(CMAG_ARRAY_POSITION, hpos.data(), 0, (int)n));
+    CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITY, zero.data(), 0, (int)n));
+    CMAG_TRY(s->set_array(CMAG_ARRAY_MEASURES, meas.data(), 0, (int)n));
+    CMAG_TRY(s->set_array(CMAG_ARRAY_ACCELERATION, zero.data(), 0, (int)n));
+    CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITYLEAPFROG, zero.data(), 0, (int)n));
+    if (s->ref_kind == CMAG_SYS_DAMBREAK) {
+        const cmag_params& P = s->P;
+        s->P.right_boundary = P.world_origin[0] + (P.boundary_offset + P.fluid_size[0]) * 2 * P.particle_radius;
+    }
+    return CMAG_OK;
+}
+
+int cmag_system_update(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    return s->update();
+}
+
+int cmag_system_update_dt(cmag_system* s, float dt) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (!(dt > 0.0f)) return fail(CMAG_E_INVALID, "update(dt): dt must be positive");
+    if (dt != s->P.delta_time) {
+        s->P.delta_time = dt;
+        s->invalidate_graph();
+    }
+    return s->update();
+}
+
+int cmag_system_run(cmag_system* s, uint32_t steps, int use_graph) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    return s->run(steps, use_graph != 0);
+}
+
+int cmag_system_set_array(cmag_system* s, int array, const float* data, int start, int count) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    return s->set_array(array, data, start, count);
+}
+
+int cmag_system_get_array(cmag_system* s, int array, float* data, int start, int count) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    return s->get_array(array, data, start, count);
+}
+
+int cmag_system_get_uint_array(cmag_system* s, int array, uint32_t* data, uint32_t start, uint32_t count) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    uint32_t len = 0;
+    uint32_t* d = s->uarray(array, &len);
+    if (!d) return fail(CMAG_E_INVALID, "unknown uint array");
+    if ((uint64_t)start + count > len) return fail(CMAG_E_INVALID, "range outside the array");
+    if (count == 0) return CMAG_OK;
+    if (!data) return fail(CMAG_E_INVALID, "null data");
+    CMAG_HIP_TRY(hipMemcpyAsync(data, d + start, sizeof(uint32_t) * (size_t)count, hipMemcpyDeviceToHost,
+                                s->stream));
+    CMAG_HIP_TRY(hipStreamSynchronize(s->stream));
+    return CMAG_OK;
+}
+
+void* cmag_system_device_ptr(cmag_system* s, int array) {
+    if (!s) return nullptr;
+    float* f = s->farray(array);
+    if (f) return f;
+    uint32_t len;
+    return s->uarray(array, &len);
+}
+
+uint32_t cmag_system_num_particles(const cmag_system* s) { return s ? s->n : 0u; }
+uint32_t cmag_system_num_cells(const cmag_system* s) { return s ? s->numCells : 0u; }
+float cmag_system_elapsed_time(const cmag_system* s) { return s ? s->elapsed : 0.0f; }
+
+int cmag_system_get_params(const cmag_system* s, cmag_params* out) {
+    if (!s || !out) return fail(CMAG_E_INVALID, "null argument");
+    *out = s->P;
+    return CMAG_OK;
+}
+
+int cmag_system_set_params(cmag_system* s, const cmag_params* p) {
+    if (!s || !p) return fail(CMAG_E_INVALID, "null argument");
+    CMAG_TRY(validate_params(p));
+    for (int a = 0; a < 3; ++a)
+        if (p->grid_size[a] != s->P.grid_size[a]) return fail(CMAG_E_INVALID, "set_params may not change the grid");
+    if (p->system != s->P.system) return fail(CMAG_E_INVALID, "set_params may not change the system kind");
+    s->P = *p;
+    s->key_bits = cmag_params_key_bits(p);
+    s->invalidate_graph();
+    return CMAG_OK;
+}
+
+int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_HIP_TRY(hipStreamSynchronize(s->stream));
+    s->stream = stream ? as_stream(stream) : s->own_stream;
+    s->invalidate_graph();
+    return CMAG_OK;
+}
+
+int cmag_system_synchronize(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_HIP_TRY(hipStreamSynchronize(s->stream));
+    return CMAG_OK;
+}
+
+// DamBreakSystem::changeRightBoundary, fluidSystem.cpp:239-255
+int cmag_system_change_right_boundary(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (s->P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_STATE, "DamBreak only");
+    s->P.right_boundary += s->P.fluid_size[0] * 2 * s->P.particle_radius;
+    s->invalidate_graph();
+    return launch_shift_right_boundary(s->P, s->pos, s->n, s->stream);
+}
+
+// DamBreakSystem::removeRightBoundary, fluidSystem.cpp:222-237
+int cmag_system_remove_right_boundary(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (s->P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_STATE, "DamBreak only");
+    s->P.right_boundary = (float)0xffffffffu;
+    s->invalidate_graph();
+    CMAG_TRY(launch_remove_right_boundary(s->P, s->pos, s->n, s->stream));
+    s->elapsed = 0.0f;
+    return CMAG_OK;
+}
+
+// PoiseuilleFlowSystem::changeGravity, poiseuilleFlowSystem.cpp:207-210
+int cmag_system_change_gravity(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    s->P.gravity[1] *= -1.0f;
+    s->invalidate_graph();
+    return CMAG_OK;
+}
+
+}  // extern "C"

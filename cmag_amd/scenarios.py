@@ -1,0 +1,352 @@
+"""Parameter blocks and initial states.
+
+Reference blocks come from the library's constructors (cmag_params_*), which
+restate the reference constructors.  Configuration scenarios (the channel and
+dam-break workloads this package is measured on) are built here:
+
+  poiseuille_channel  2-D periodic channel, Tait EOS, 3 wall layers, seeded jitter
+  dambreak3d          3-D water column in a tank with 3-layer walls, open top
+
+Particle mass in configuration mode is rho0 over the lattice kernel sum (the
+method of PeristalsisSystem::CalculateMass, PeristalsisSystem.cpp:238-253,
+taken over an unbounded lattice), so an interior particle of the unperturbed
+lattice has density rho0.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import CmagParams, call
+
+PI_F = np.float32(3.141592654)
+
+
+# ---------------------------------------------------------------- reference
+def params_dambreak(fluid, bo, grid, r) -> CmagParams:
+    p = CmagParams()
+    call("cmag_params_dambreak", C.byref(p), *map(int, fluid), int(bo), *map(int, grid), C.c_float(r))
+    return p
+
+
+def params_poiseuille(fluid, bo, grid, r) -> CmagParams:
+    p = CmagParams()
+    call("cmag_params_poiseuille", C.byref(p), *map(int, fluid), int(bo), *map(int, grid), C.c_float(r))
+    return p
+
+
+def params_peristalsis(dt, fluid, amplitude, wave_speed, soundspeed, gravity, bo, grid, r) -> CmagParams:
+    p = CmagParams()
+    call("cmag_params_peristalsis", C.byref(p), C.c_float(dt), *map(int, fluid), C.c_float(amplitude),
+         C.c_float(wave_speed), C.c_float(soundspeed), *[C.c_float(float(g)) for g in gravity], int(bo),
+         *map(int, grid), C.c_float(r))
+    return p
+
+
+def finalize(p: CmagParams) -> CmagParams:
+    call("cmag_params_finalize", C.byref(p))
+    return p
+
+
+def num_cells(p: CmagParams) -> int:
+    return int(p.grid_size[0]) * int(p.grid_size[1]) * int(p.grid_size[2])
+
+
+def key_bits(p: CmagParams) -> int:
+    return int(p.grid_bits[0] + p.grid_bits[1] + p.grid_bits[2])
+
+
+# ---------------------------------------------------------------- utilities
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014); uniform floats from the top 24 bits."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, n: int) -> np.ndarray:
+        """n doubles in [0,1) (vectorised splitmix64 over consecutive states)."""
+        k = np.arange(1, n + 1, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            z = np.uint64(self.state) + k * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.state = (self.state + n * 0x9E3779B97F4A7C15) & self.MASK
+        return (z >> np.uint64(40)).astype(np.float64) / float(1 << 24)
+
+
+def wendland_w(q: np.ndarray, h: float, dim: int) -> np.ndarray:
+    """Wendland C2 kernel W(q) with the 2-D 7/(4 pi h^2) or 3-D 21/(16 pi h^3) factor."""
+    a = 7.0 / (4.0 * math.pi * h * h) if dim == 2 else 21.0 / (16.0 * math.pi * h ** 3)
+    q = np.asarray(q, dtype=np.float64)
+    return np.where(q < 2.0, a * (1.0 - 0.5 * q) ** 4 * (2.0 * q + 1.0), 0.0)
+
+
+def lattice_kernel_sum(h: float, dx: float, dim: int) -> float:
+    """sum_j W(|x_j|) over an unbounded square/cubic lattice of spacing dx (FP64)."""
+    k = int(math.ceil(2.0 * h / dx)) + 1
+    r = np.arange(-k, k + 1, dtype=np.float64) * dx
+    if dim == 2:
+        X, Y = np.meshgrid(r, r, indexing="ij")
+        d = np.sqrt(X * X + Y * Y)
+    else:
+        X, Y, Z = np.meshgrid(r, r, r, indexing="ij")
+        d = np.sqrt(X * X + Y * Y + Z * Z)
+    return float(wendland_w(d / h, h, dim).sum())
+
+
+def lattice_laplacian_moment(h: float, dx: float, dim: int) -> float:
+    """-(dx^dim / dim) sum_j |x_j| W'(|x_j|) over the lattice: the factor by
+    which the Morris viscous sum reproduces nu * laplacian(u) on that lattice
+    (1 in the continuum limit; 0.974 at h = 1.3 dx in 2-D)."""
+    k = int(math.ceil(2.0 * h / dx)) + 1
+    r = np.arange(-k, k + 1, dtype=np.float64) * dx
+    grids = np.meshgrid(*([r] * dim), indexing="ij")
+    d = np.sqrt(sum(g * g for g in grids))
+    q = d / h
+    a = 7.0 / (2.0 * math.pi * h ** 3) if dim == 2 else 21.0 / (8.0 * math.pi * h ** 4)
+    t = np.clip(1.0 - 0.5 * q, 0.0, None)
+    dw = np.where(q < 2.0, a * (t ** 4 - t ** 3 * (2.0 * q + 1.0)), 0.0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        term = np.where(d > 0, d * dw, 0.0)
+    return float(-term.sum() * dx ** dim / dim)
+
+
+def _pow2_at_least(v: float) -> int:
+    n = 1
+    while n < v:
+        n *= 2
+    return n
+
+
+def _blank(system: int, dim: int) -> CmagParams:
+    p = CmagParams()
+    p.version = 1
+    p.system = system
+    p.dim = dim
+    p.hash_order = L.HASH_LINEAR
+    p.gridpos_rule = L.GRIDPOS_DIV_CELL
+    p.boundary_damping = -1.0
+    p.alpha = 0.38
+    p.beta_max = 1000.0
+    p.gamma = 7.0
+    p.rest_density = 1000.0
+    inf = float("inf")
+    for a in range(3):
+        p.clamp_lo[a] = -inf
+        p.clamp_hi[a] = inf
+    p.right_boundary = inf
+    return p
+
+
+# ---------------------------------------------------------------- channel
+@dataclass
+class Scenario:
+    params: CmagParams
+    pos: np.ndarray        # (N, 4) float32, w = particle type
+    vel: np.ndarray        # (N, 4) float32
+    num_fluid: int
+    meta: dict
+
+
+def poiseuille_channel(nx: int = 32, ny: int = 64, width: float = 1.0, bo: int = 3,
+                       nu: float = 1e-2, vmax: float = 1e-2, h_factor: float = 1.3,
+                       jitter: float = 1e-3, seed: int = 1973, eos: int = L.EOS_TAIT,
+                       cellcount: int | None = None, cell_factor: float = 1.0,
+                       gz: int = 4, gravity: float | None = None) -> Scenario:
+    """2-D periodic Poiseuille channel in the Poiseuille system's loop bodies.
+
+    Fluid nx x ny at spacing dx = width/ny, periodic in x with length nx*dx;
+    bo wall layers (w = 1) above and below.  Body force g = 8 nu vmax / width^2,
+    sound speed c0 = 10 vmax, Tait gamma 7, dt = min(0.25 h/c0, 0.125 h^2/nu).
+    Cells are cell_factor*dx wide along x (nx/cell_factor must be a power of
+    two so the grid spans the period exactly)."""
+    dx = width / ny
+    r = dx / 2.0
+    h = h_factor * dx
+    cs = cell_factor * dx
+    gx = int(round(nx / cell_factor))
+    if gx * cell_factor != nx or gx & (gx - 1):
+        raise ValueError("nx / cell_factor must be a power of two")
+    if cellcount is None:
+        cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
+    height = (ny + 2 * bo) * dx
+    gy = _pow2_at_least(height / cs + 2)
+    p = _blank(L.SYS_POISEUILLE, 2)
+    p.eos = eos
+    p.periodic_mask = 1
+    p.cellcount = cellcount
+    for a, g in enumerate((gx, gy, gz)):
+        p.grid_size[a] = g
+    p.fluid_size[0], p.fluid_size[1], p.fluid_size[2] = nx, ny, 1
+    p.particle_radius = r
+    p.smoothing_radius = h
+    p.cell_size[0] = p.cell_size[1] = p.cell_size[2] = cs
+    # world: x periodic over [-nx dx/2, nx dx/2); y from the bottom of the walls
+    p.world_origin[0] = np.float32(-0.5 * nx * dx)
+    p.world_origin[1] = np.float32(-0.5 * gy * cs)
+    p.world_origin[2] = np.float32(-0.5 * gz * cs)
+    p.world_size[0] = np.float32(gx) * np.float32(cs)
+    p.world_size[1] = np.float32(gy) * np.float32(cs)
+    p.world_size[2] = np.float32(gz) * np.float32(cs)
+    c0 = 10.0 * vmax
+    p.soundspeed = c0
+    p.B = 1000.0 * c0 * c0 / 7.0
+    p.mu = 1000.0 * nu
+    g = 8.0 * nu * vmax / (width * width) if gravity is None else gravity
+    p.gravity[0], p.gravity[1], p.gravity[2] = g, 0.0, 0.0
+    p.delta_time = min(0.25 * h / c0, 0.125 * h * h / nu)
+    p.particle_mass = 1000.0 / lattice_kernel_sum(h, dx, 2)
+    # channel: fluid rows y0 + (j+1/2) dx for j in [0, ny); walls below/above
+    y0 = np.float32(-0.5 * height + bo * dx)
+    p.wall_bottom = y0
+    p.wall_top = np.float32(y0 + ny * dx)
+    finalize(p)
+
+    rng = SplitMix64(seed)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    fx = (np.float64(p.world_origin[0]) + (ii.ravel() + 0.5) * dx)
+    fy = (np.float64(y0) + (jj.ravel() + 0.5) * dx)
+    nf = nx * ny
+    jit = (rng.uniform(2 * nf).reshape(2, nf) * 2.0 - 1.0) * jitter * dx
+    fluid = np.zeros((nf, 4), np.float64)
+    fluid[:, 0] = fx + jit[0]
+    fluid[:, 1] = fy + jit[1]
+    fluid[:, 2] = float(p.world_origin[2]) + 0.5 * cs
+    wi, wj = np.meshgrid(np.arange(nx), np.arange(bo), indexing="xy")
+    wx = np.float64(p.world_origin[0]) + (wi.ravel() + 0.5) * dx
+    bottom = np.zeros((nx * bo, 4)); top = np.zeros((nx * bo, 4))
+    bottom[:, 0] = wx
+    bottom[:, 1] = np.float64(y0) - (wj.ravel() + 0.5) * dx
+    top[:, 0] = wx
+    top[:, 1] = np.float64(y0) + ny * dx + (wj.ravel() + 0.5) * dx
+    bottom[:, 2] = top[:, 2] = fluid[0, 2]
+    bottom[:, 3] = top[:, 3] = 1.0
+    pos = np.concatenate([fluid, bottom, top]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    meta = dict(dx=dx, h=h, nu=nu, vmax=vmax, width=width, g=g, y0=float(y0), nx=nx, ny=ny)
+    return Scenario(p, pos, vel, nf, meta)
+
+
+def poiseuille_series(y: np.ndarray, t: float, F: float, l: float, nu: float,
+                      nterms: int = 101) -> np.ndarray:
+    """Transient start-up Poiseuille profile, the series of
+    Poiseuille.Report/XVelocityYPosition.m:12 (the script plots -Vel; this is the
+    positive velocity):  u = F y (l-y)/(2 nu) - sum_n 4 F l^2 sin((2n+1) pi y/l)
+    exp(-(2n+1)^2 pi^2 nu t / l^2) / (nu pi^3 (2n+1)^3)."""
+    y = np.asarray(y, dtype=np.float64)
+    u = F * y * (l - y) / (2.0 * nu)
+    for n in range(nterms):
+        k = 2 * n + 1
+        u -= (4.0 * F * l * l * np.sin(k * math.pi * y / l) * math.exp(-k * k * math.pi ** 2 * nu * t / (l * l))
+              / (nu * math.pi ** 3 * k ** 3))
+    return u
+
+
+# ---------------------------------------------------------------- dam break
+def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), layers: int = 3,
+               h_factor: float = 1.3, alpha: float = 0.01, jitter: float = 0.01, seed: int = 1973,
+               grid=None, hash_order: int = L.HASH_LINEAR, cell_factor: float = 1.0) -> Scenario:
+    """3-D dam break in the DamBreak system's loop bodies (Tait EOS, Monaghan
+    artificial viscosity, Lennard-Jones wall particles, clamp walls).
+
+    Water column column[0] x column[1] x column[2] against the x=0 wall of a
+    tank tank[0] x tank[1] x tank[2]; `layers` wall layers on the floor and the
+    four sides (open top); lattice spacing dx with seeded jitter +-jitter*dx.
+    Cells are cell_factor * 2h wide (cellcount = ceil(2h / cell))."""
+    h = h_factor * dx
+    cs = cell_factor * 2.0 * h
+    cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
+    nfx, nfy, nfz = (int(round(c / dx)) for c in column)
+    ext = [t + 2 * layers * dx for t in tank]
+    if grid is None:
+        grid = tuple(_pow2_at_least(e / cs + 1) for e in ext)
+    p = _blank(L.SYS_DAMBREAK, 3)
+    p.hash_order = hash_order
+    p.eos = L.EOS_TAIT
+    p.cellcount = cellcount
+    for a in range(3):
+        p.grid_size[a] = grid[a]
+        p.cell_size[a] = cs
+        p.world_origin[a] = np.float32(-layers * dx - 0.5 * (grid[a] * cs - ext[a]))
+        p.world_size[a] = np.float32(grid[a] * cs)
+    p.fluid_size[0], p.fluid_size[1], p.fluid_size[2] = nfx, nfy, nfz
+    p.particle_radius = dx / 2.0
+    p.smoothing_radius = h
+    g = 9.81
+    H = column[1]
+    c0 = 10.0 * math.sqrt(2.0 * g * H)
+    p.soundspeed = c0
+    p.gamma = 7.0
+    p.B = 1000.0 * c0 * c0 / 7.0
+    p.alpha = alpha
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -g, 0.0
+    p.delta_time = 0.25 * h / c0
+    p.particle_mass = 1000.0 / lattice_kernel_sum(h, dx, 3)
+    # Lennard-Jones walls scaled as the reference scales them (fluidSystem.cpp:70-71)
+    p.lj_D = 10.0 * (-g) * H
+    p.lj_a = dx / 2.0
+    # clamp box = tank interior
+    p.clamp_lo[0], p.clamp_lo[1], p.clamp_lo[2] = 0.0, 0.0, 0.0
+    p.right_boundary = tank[0]
+    p.clamp_hi[1] = float("inf")
+    p.clamp_hi[2] = tank[2]
+    p.boundary_offset = layers
+    finalize(p)
+
+    rng = SplitMix64(seed)
+    I, J, K = np.meshgrid(np.arange(nfx), np.arange(nfy), np.arange(nfz), indexing="ij")
+    nf = I.size
+    # lattice order x fastest, then y, then z (as initFluid)
+    I = I.transpose(2, 1, 0).ravel(); J = J.transpose(2, 1, 0).ravel(); K = K.transpose(2, 1, 0).ravel()
+    jit = (rng.uniform(3 * nf).reshape(3, nf) * 2.0 - 1.0) * jitter * dx
+    fluid = np.empty((nf, 4), np.float64)
+    fluid[:, 0] = (I + 0.5) * dx + jit[0]
+    fluid[:, 1] = (J + 0.5) * dx + jit[1]
+    fluid[:, 2] = (K + 0.5) * dx + jit[2]
+    fluid[:, 3] = 4.0  # DamBreak fluid type
+    # walls: floor and four sides, `layers` deep, spacing dx, outside the tank
+    nx_t, ny_t, nz_t = (int(round(t / dx)) for t in tank)
+    xs = (np.arange(-layers, nx_t + layers) + 0.5) * dx
+    zs = (np.arange(-layers, nz_t + layers) + 0.5) * dx
+    ys = (np.arange(0, ny_t) + 0.5) * dx
+    walls = []
+    for k in range(layers):
+        y = -(k + 0.5) * dx
+        X, Z = np.meshgrid(xs, zs, indexing="ij")
+        walls.append(np.stack([X.ravel(), np.full(X.size, y), Z.ravel()], 1))
+    side_x = np.concatenate([(-np.arange(1, layers + 1) + 0.5) * dx, (nx_t + np.arange(layers) + 0.5) * dx])
+    X, Y, Z = np.meshgrid(side_x, ys, zs, indexing="ij")
+    walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
+    inner_x = (np.arange(0, nx_t) + 0.5) * dx
+    side_z = np.concatenate([(-np.arange(1, layers + 1) + 0.5) * dx, (nz_t + np.arange(layers) + 0.5) * dx])
+    X, Y, Z = np.meshgrid(inner_x, ys, side_z, indexing="ij")
+    walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
+    w = np.concatenate(walls)
+    wall = np.empty((w.shape[0], 4), np.float64)
+    wall[:, :3] = w
+    wall[:, 3] = 1.0  # DamBreak FirstType wall
+    pos = np.concatenate([fluid, wall]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    meta = dict(dx=dx, h=h, c0=c0, tank=tank, column=column, num_wall=int(wall.shape[0]))
+    return Scenario(p, pos, vel, nf, meta)
+
+
+def dambreak3d_particles(target_fluid: int, **kw) -> Scenario:
+    """Dam break with the lattice spacing chosen for about target_fluid fluid particles
+    (0.4 x 0.8 x 0.4 column, dx = 0.4 / round((target/2)^(1/3)))."""
+    n = int(round((target_fluid / 2.0) ** (1.0 / 3.0)))
+    return dambreak3d(dx=0.4 / n, **kw)

@@ -1,0 +1,217 @@
+/*
+ * cmag_hip.h - C ABI of libcmag_hip.so, the MI355X WCSPH timestep.
+ *
+ * Plain C: device pointers, counts, a parameter block (cmag_params.h) and a
+ * stream handle; every entry point returns an int status (CMAG_OK = 0).
+ * Launchers are asynchronous on the given stream (NULL = the default stream)
+ * and never synchronise; host-array calls (set/get array) synchronise the
+ * object's stream.  Errors never exit the process: cmag_last_error() returns
+ * the message of the last failure on the calling thread.
+ *
+ * Each launcher replaces one extern "C" launcher of the reference's
+ * System*.cu files (cited per entry).  The reference reused the same symbol
+ * names in all three libraries (sortParticles, setParameters, ...) and kept
+ * parameters in a process-global __constant__; here names carry a cmag_
+ * prefix, the system is selected by cmag_params.system, and the parameter
+ * block travels with every call, so instances and systems can coexist.
+ */
+#ifndef CMAG_HIP_H
+#define CMAG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "cmag_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* cmag_stream_t;             /* a hipStream_t */
+typedef struct cmag_system cmag_system;  /* opaque System object */
+
+enum cmag_status {
+    CMAG_OK = 0,
+    CMAG_E_INVALID = -1, /* bad argument (null pointer, size, grid not a power of two) */
+    CMAG_E_HIP = -2,     /* a HIP runtime call failed */
+    CMAG_E_SIZE = -3,    /* scratch or workspace too small */
+    CMAG_E_STATE = -4    /* call not valid in the object's current state */
+};
+
+/* particle arrays of a System object (float4 per particle) */
+enum cmag_array {
+    CMAG_ARRAY_POSITION = 0,
+    CMAG_ARRAY_VELOCITY = 1,
+    CMAG_ARRAY_MEASURES = 2,         /* (rho, p, ., .) in sorted order */
+    CMAG_ARRAY_ACCELERATION = 3,     /* DamBreak/Poiseuille acceleration; Peristalsis viscous force */
+    CMAG_ARRAY_VELOCITYLEAPFROG = 4,
+    CMAG_ARRAY_PRESSUREFORCE = 5,    /* Peristalsis only */
+    CMAG_ARRAY_SORTEDPOS = 6,
+    CMAG_ARRAY_SORTEDVEL = 7,
+    CMAG_ARRAY_PREDICTEDPOSITION = 8,/* Peristalsis only (kept for setArray parity) */
+    /* uint32 arrays */
+    CMAG_ARRAY_HASH = 16,
+    CMAG_ARRAY_INDEX = 17,
+    CMAG_ARRAY_CELLSTART = 18,
+    CMAG_ARRAY_CELLEND = 19
+};
+
+/* ---- library ---------------------------------------------------------- */
+const char* cmag_last_error(void);
+int cmag_version(void);
+/* sizeof(cmag_params) as compiled into the library (binding check) */
+int cmag_params_size(void);
+
+/* ---- parameter blocks (host only, no device work) ------------------------ */
+/* DamBreakSystem ctor, DamBreak.Core/fluidSystem.cpp:17-77 */
+int cmag_params_dambreak(cmag_params* out, uint32_t fx, uint32_t fy, uint32_t fz,
+                         int32_t boundaryOffset, uint32_t gx, uint32_t gy, uint32_t gz,
+                         float particleRadius);
+/* PoiseuilleFlowSystem ctor, Poiseuille.Core/poiseuilleFlowSystem.cpp:21-67 */
+int cmag_params_poiseuille(cmag_params* out, uint32_t fx, uint32_t fy, uint32_t fz,
+                           int32_t boundaryOffset, uint32_t gx, uint32_t gy, uint32_t gz,
+                           float particleRadius);
+/* PeristalsisSystem ctor, Peristalsis.Core/PeristalsisSystem.cpp:17-80 (mass is
+ * set by cmag_system_reset from the lattice, PeristalsisSystem.cpp:232) */
+int cmag_params_peristalsis(cmag_params* out, float deltaTime, uint32_t fx, uint32_t fy,
+                            uint32_t fz, float amplitude, float wave_speed, float soundspeed,
+                            float gravx, float gravy, float gravz, int32_t boundaryOffset,
+                            uint32_t gx, uint32_t gy, uint32_t gz, float radius);
+/* recompute the derived fields (grid_bits, Morton masks, kernel coefficients,
+ * wall planes, clamp borders) after editing a block by hand */
+int cmag_params_finalize(cmag_params* p);
+uint32_t cmag_params_num_cells(const cmag_params* p);
+/* number of key bits of the cell hash: ceil(log2(num_cells)) */
+int cmag_params_key_bits(const cmag_params* p);
+
+/* the reference reset() lattice for p->system (host memory, no device work):
+ * DamBreakSystem::initFluid/initBoundaryParticles fluidSystem.cpp:382-523,
+ * PoiseuilleFlowSystem's poiseuilleFlowSystem.cpp:325-383,
+ * PeristalsisSystem's PeristalsisSystem.cpp:255-317 */
+int cmag_reference_initial_state(const cmag_params* p, uint32_t n, float* pos4);
+/* PeristalsisSystem::CalculateMass's kernel sum, PeristalsisSystem.cpp:238-253 */
+float cmag_peristalsis_lattice_sum(const cmag_params* p, const float* pos4, uint32_t n);
+
+/* ---- stage launchers (device pointers; asynchronous) --------------------- */
+/* calcHash: fluidSystem.cu:133-148 (+ poiseuilleFlowSystem.cu:110-125,
+ * PeristalsisSystem.cu:99-114).  hash[i] = cell key of pos[i], index[i] = i. */
+int cmag_calc_hash(const cmag_params* p, const float* pos4, uint32_t* hash,
+                   uint32_t* index, uint32_t n, cmag_stream_t stream);
+
+/* sortParticles: fluidSystem.cu:94-99 (thrust::sort_by_key).  Stable
+ * ascending LSD radix sort of keys < 2^key_bits, in place, with a scratch
+ * buffer of at least cmag_sort_scratch_bytes(n, key_bits) bytes. */
+size_t cmag_sort_scratch_bytes(uint32_t n, int key_bits);
+int cmag_sort_by_key(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, void* scratch,
+                     size_t scratch_bytes, cmag_stream_t stream);
+
+/* reorderDataAndFindCellStart: fluidSystem.cu:150-188 (incl. its cudaMemset of
+ * cellStart).  cellEnd of empty cells is not written. */
+int cmag_reorder_find_cell_start(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos,
+                                 float* sortedVel, const uint32_t* hash, const uint32_t* index,
+                                 const float* oldPos, const float* oldVel, uint32_t n,
+                                 uint32_t numCells, cmag_stream_t stream);
+
+/* density + equation of state, by p->system:
+ *   calculateDamBreakDensity    fluidSystem.cu:190-230
+ *   calculatePoiseuilleDensity  poiseuilleFlowSystem.cu:167-204
+ *   computeDensityVariation     PeristalsisSystem.cu:163-199 */
+int cmag_density(const cmag_params* p, float* measures, const float* sortedPos,
+                 const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                 cmag_stream_t stream);
+
+/* fused force pass scattered to acc[index[i]]:
+ *   calcAndApplyAcceleration         fluidSystem.cu:232-272
+ *   calculatePoiseuilleAcceleration  poiseuilleFlowSystem.cu:206-247 */
+int cmag_force(const cmag_params* p, float* acc, const float* measures, const float* sortedPos,
+               const float* sortedVel, const uint32_t* index, const uint32_t* cellStart,
+               const uint32_t* cellEnd, uint32_t n, cmag_stream_t stream);
+
+/* computePressureForce, PeristalsisSystem.cu:245-283 */
+int cmag_pressure_force(const cmag_params* p, float* presF, const float* measures,
+                        const float* sortedPos, const uint32_t* index,
+                        const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                        cmag_stream_t stream);
+
+/* computeViscousForce, PeristalsisSystem.cu:201-243 */
+int cmag_viscous_force(const cmag_params* p, float* viscF, const float* measures,
+                       const float* sortedPos, const float* sortedVel, const uint32_t* index,
+                       const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                       cmag_stream_t stream);
+
+/* integrate, by p->system (presF and wave_time are Peristalsis only):
+ *   integrateSystem            fluidSystem.cu:114-131
+ *   integratePoiseuilleSystem  poiseuilleFlowSystem.cu:84-101
+ *   computeCoordinates         PeristalsisSystem.cu:308-329 */
+int cmag_integrate(const cmag_params* p, float* pos, float* vel, float* velLF, const float* acc,
+                   const float* presF, float wave_time, uint32_t n, cmag_stream_t stream);
+
+/* ExtConfigureBoundary, PeristalsisSystem.cu:84-97 */
+int cmag_configure_boundary(const cmag_params* p, float* pos, float wave_height, uint32_t n,
+                            cmag_stream_t stream);
+/* ExtChangeRightBoundary, fluidSystem.cu:81-92 */
+int cmag_shift_right_boundary(const cmag_params* p, float* pos, uint32_t n, cmag_stream_t stream);
+/* ExtRemoveRightBoundary, fluidSystem.cu:101-112 */
+int cmag_remove_right_boundary(const cmag_params* p, float* pos, uint32_t n, cmag_stream_t stream);
+
+/* ---- System objects: the reference classes behind a C handle ------------- */
+/* DamBreakSystem(fluidParticlesSize, boundaryOffset, gridSize, particleRadius),
+ * DamBreak.Core/fluidSystem.h:9-14 */
+int cmag_dambreak_create(uint32_t fx, uint32_t fy, uint32_t fz, int32_t boundaryOffset,
+                         uint32_t gx, uint32_t gy, uint32_t gz, float particleRadius,
+                         cmag_system** out);
+/* PoiseuilleFlowSystem(...), Poiseuille.Core/poiseuilleFlowSystem.h:9-14 */
+int cmag_poiseuille_create(uint32_t fx, uint32_t fy, uint32_t fz, int32_t boundaryOffset,
+                           uint32_t gx, uint32_t gy, uint32_t gz, float particleRadius,
+                           cmag_system** out);
+/* PeristalsisSystem(...), Peristalsis.Core/PeristalsisSystem.h:9-19 */
+int cmag_peristalsis_create(float deltaTime, uint32_t fx, uint32_t fy, uint32_t fz,
+                            float amplitude, float wave_speed, float soundspeed, float gravx,
+                            float gravy, float gravz, int32_t boundaryOffset, uint32_t gx,
+                            uint32_t gy, uint32_t gz, float radius, cmag_system** out);
+/* a System on an arbitrary parameter block with n particles; the caller fills
+ * the arrays with cmag_system_set_array (reset() is not available) */
+int cmag_system_create(const cmag_params* p, uint32_t n, cmag_system** out);
+int cmag_system_destroy(cmag_system* s);
+
+/* reset(): fluidSystem.cpp:361-380, poiseuilleFlowSystem.cpp:311-323,
+ * PeristalsisSystem.cpp:213-237 (Reset) */
+int cmag_system_reset(cmag_system* s);
+/* update(): fluidSystem.cpp:257-319, poiseuilleFlowSystem.cpp:212-270,
+ * PeristalsisSystem.cpp:320-407 (Update) */
+int cmag_system_update(cmag_system* s);
+/* update(dt): sets the time step, then update() */
+int cmag_system_update_dt(cmag_system* s, float dt);
+/* steps consecutive updates; DamBreak and Poiseuille replay one captured
+ * HIP graph per step when use_graph is non-zero */
+int cmag_system_run(cmag_system* s, uint32_t steps, int use_graph);
+
+/* setArray(array, data, start, count), fluidSystem.cpp:321-355: count
+ * particles of 4 floats from host memory, starting at particle `start` */
+int cmag_system_set_array(cmag_system* s, int array, const float* data, int start, int count);
+/* getArray: the device-to-host counterpart (not in the reference, whose
+ * harnesses copied from the raw device-pointer getters) */
+int cmag_system_get_array(cmag_system* s, int array, float* data, int start, int count);
+int cmag_system_get_uint_array(cmag_system* s, int array, uint32_t* data, uint32_t start,
+                               uint32_t count);
+/* the raw device pointer getters, fluidSystem.h:40-47 (borrowed) */
+void* cmag_system_device_ptr(cmag_system* s, int array);
+
+uint32_t cmag_system_num_particles(const cmag_system* s);
+uint32_t cmag_system_num_cells(const cmag_system* s);
+float cmag_system_elapsed_time(const cmag_system* s);
+int cmag_system_get_params(const cmag_system* s, cmag_params* out);
+/* replaces the block (the setParameters of the reference); a new block may
+ * not change the particle count or the grid */
+int cmag_system_set_params(cmag_system* s, const cmag_params* p);
+int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream);
+int cmag_system_synchronize(cmag_system* s);
+
+/* scenario controls: fluidSystem.cpp:222-255, poiseuilleFlowSystem.cpp:207-210 */
+int cmag_system_change_right_boundary(cmag_system* s);
+int cmag_system_remove_right_boundary(cmag_system* s);
+int cmag_system_change_gravity(cmag_system* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMAG_HIP_H */

@@ -1,0 +1,285 @@
+"""ctypes wrapper of oracle/liboracle.so.  TEST INFRASTRUCTURE ONLY.
+
+The oracle is the checker: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use it, never the product path.  See cmag_oracle.c for
+what each restated function follows in the reference and how it is pinned.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    proc = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.oracle_grid_hash.restype = C.c_uint32
+        _lib.oracle_dambreak_count.restype = C.c_uint32
+        _lib.oracle_peristalsis_mass_sum.restype = C.c_float
+        _lib.oracle_get_threads.restype = C.c_int
+    return _lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().oracle_get_threads())
+
+
+def grid_pos(P, x: float, y: float, z: float) -> tuple[int, int, int]:
+    out = (C.c_int32 * 3)()
+    lib().oracle_grid_pos(C.byref(P), C.c_float(x), C.c_float(y), C.c_float(z), out)
+    return tuple(out)
+
+
+def grid_hash(P, g) -> int:
+    arr = (C.c_int32 * 3)(*[int(v) for v in g])
+    return int(lib().oracle_grid_hash(C.byref(P), arr))
+
+
+def calc_hash(P, pos4: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    pos4 = _f(pos4, np.float32)
+    n = pos4.shape[0]
+    h = np.empty(n, np.uint32)
+    idx = np.empty(n, np.uint32)
+    lib().oracle_calc_hash(C.byref(P), _ptr(pos4), _ptr(h), _ptr(idx), C.c_uint32(n))
+    return h, idx
+
+
+def sort_by_key(keys: np.ndarray, vals: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    k = np.array(keys, dtype=np.uint32, copy=True)
+    v = np.array(vals, dtype=np.uint32, copy=True)
+    lib().oracle_sort_by_key(_ptr(k), _ptr(v), C.c_uint32(k.size))
+    return k, v
+
+
+def find_cell_start(sorted_hash: np.ndarray, num_cells: int,
+                    cell_end_init: int = 0) -> tuple[np.ndarray, np.ndarray]:
+    sh = _f(sorted_hash, np.uint32)
+    cs = np.empty(num_cells, np.uint32)
+    ce = np.full(num_cells, cell_end_init, np.uint32)
+    lib().oracle_find_cell_start(_ptr(cs), _ptr(ce), _ptr(sh), C.c_uint32(sh.size), C.c_uint32(num_cells))
+    return cs, ce
+
+
+def _sfx(dtype) -> str:
+    return "_f32" if np.dtype(dtype) == np.float32 else "_f64"
+
+
+def gather(index, pos, vel, dtype=np.float32):
+    pos = _f(pos, dtype); vel = _f(vel, dtype)
+    n = pos.shape[0]
+    sp = np.empty_like(pos); sv = np.empty_like(vel)
+    getattr(lib(), "oracle_gather" + _sfx(dtype))(_ptr(sp), _ptr(sv), _ptr(_f(index, np.uint32)),
+                                                 _ptr(pos), _ptr(vel), C.c_uint32(n))
+    return sp, sv
+
+
+def density(P, sorted_pos, cell_start, cell_end, measures_in=None, dtype=np.float32):
+    sp = _f(sorted_pos, dtype)
+    n = sp.shape[0]
+    meas = np.zeros((n, 4), dtype) if measures_in is None else np.array(measures_in, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_density" + _sfx(dtype))(C.byref(P), _ptr(meas), _ptr(sp),
+                                                  _ptr(_f(cell_start, np.uint32)),
+                                                  _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    return meas
+
+
+def force(P, measures, sorted_pos, sorted_vel, index, cell_start, cell_end, acc_in=None,
+          dtype=np.float32):
+    sp = _f(sorted_pos, dtype)
+    n = sp.shape[0]
+    acc = np.zeros((n, 4), dtype) if acc_in is None else np.array(acc_in, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_force" + _sfx(dtype))(C.byref(P), _ptr(acc), _ptr(_f(measures, dtype)), _ptr(sp),
+                                                _ptr(_f(sorted_vel, dtype)), _ptr(_f(index, np.uint32)),
+                                                _ptr(_f(cell_start, np.uint32)), _ptr(_f(cell_end, np.uint32)),
+                                                C.c_uint32(n))
+    return acc
+
+
+def pressure_force(P, measures, sorted_pos, index, cell_start, cell_end, dtype=np.float32):
+    sp = _f(sorted_pos, dtype)
+    n = sp.shape[0]
+    out = np.zeros((n, 4), dtype)
+    getattr(lib(), "oracle_pressure_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(_f(measures, dtype)),
+                                                         _ptr(sp), _ptr(_f(index, np.uint32)),
+                                                         _ptr(_f(cell_start, np.uint32)),
+                                                         _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    return out
+
+
+def viscous_force(P, measures, sorted_pos, sorted_vel, index, cell_start, cell_end, dtype=np.float32):
+    sp = _f(sorted_pos, dtype)
+    n = sp.shape[0]
+    out = np.zeros((n, 4), dtype)
+    getattr(lib(), "oracle_viscous_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(_f(measures, dtype)),
+                                                        _ptr(sp), _ptr(_f(sorted_vel, dtype)),
+                                                        _ptr(_f(index, np.uint32)),
+                                                        _ptr(_f(cell_start, np.uint32)),
+                                                        _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    return out
+
+
+def integrate(P, pos, vel, vel_lf, acc, pres_f=None, wave_time=0.0, dtype=np.float32):
+    pos = np.array(pos, dtype=dtype, copy=True)
+    vel = np.array(vel, dtype=dtype, copy=True)
+    vlf = np.array(vel_lf, dtype=dtype, copy=True)
+    acc = _f(acc, dtype)
+    pf = None if pres_f is None else _f(pres_f, dtype)
+    R = C.c_float if np.dtype(dtype) == np.float32 else C.c_double
+    getattr(lib(), "oracle_integrate" + _sfx(dtype))(C.byref(P), _ptr(pos), _ptr(vel), _ptr(vlf), _ptr(acc),
+                                                    _ptr(pf), R(wave_time), C.c_uint32(pos.shape[0]))
+    return pos, vel, vlf
+
+
+def configure_boundary(P, pos, wave_height, dtype=np.float32):
+    pos = np.array(pos, dtype=dtype, copy=True)
+    R = C.c_float if np.dtype(dtype) == np.float32 else C.c_double
+    getattr(lib(), "oracle_configure_boundary" + _sfx(dtype))(C.byref(P), _ptr(pos), R(wave_height),
+                                                             C.c_uint32(pos.shape[0]))
+    return pos
+
+
+def shift_right_boundary(P, pos, dtype=np.float32):
+    pos = np.array(pos, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_shift_right_boundary" + _sfx(dtype))(C.byref(P), _ptr(pos), C.c_uint32(pos.shape[0]))
+    return pos
+
+
+def remove_right_boundary(P, pos, dtype=np.float32):
+    pos = np.array(pos, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_remove_right_boundary" + _sfx(dtype))(C.byref(P), _ptr(pos), C.c_uint32(pos.shape[0]))
+    return pos
+
+
+class _State32(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("pos", "vel", "velLF", "acc", "presF", "measures",
+                                          "sortedPos", "sortedVel", "hash", "index", "cellStart",
+                                          "cellEnd")] + \
+               [("n", C.c_uint32), ("numCells", C.c_uint32),
+                ("elapsed", C.c_float), ("time_shift", C.c_float), ("time_relax", C.c_float),
+                ("wave_height", C.c_float)]
+
+
+class _State64(C.Structure):
+    _fields_ = _State32._fields_[:14] + [("elapsed", C.c_double), ("time_shift", C.c_double),
+                                         ("time_relax", C.c_double), ("wave_height", C.c_double)]
+
+
+class Simulation:
+    """update() of the oracle over owned numpy arrays (one System object)."""
+
+    def __init__(self, P, pos, vel=None, vel_lf=None, measures=None, dtype=np.float32):
+        self.P = P.copy() if hasattr(P, "copy") else P
+        self.dtype = np.dtype(dtype)
+        n = np.asarray(pos).shape[0]
+        self.n = n
+        self.num_cells = int(P.grid_size[0]) * int(P.grid_size[1]) * int(P.grid_size[2])
+        z = lambda: np.zeros((n, 4), dtype)  # noqa: E731
+        self.pos = np.array(pos, dtype=dtype, copy=True)
+        self.vel = z() if vel is None else np.array(vel, dtype=dtype, copy=True)
+        self.velLF = z() if vel_lf is None else np.array(vel_lf, dtype=dtype, copy=True)
+        self.acc = z()
+        self.presF = z()
+        if measures is None:
+            self.measures = z()
+            self.measures[:, 0] = P.rest_density
+        else:
+            self.measures = np.array(measures, dtype=dtype, copy=True)
+        self.sortedPos = z()
+        self.sortedVel = z()
+        self.hash = np.zeros(n, np.uint32)
+        self.index = np.zeros(n, np.uint32)
+        self.cellStart = np.zeros(self.num_cells, np.uint32)
+        self.cellEnd = np.zeros(self.num_cells, np.uint32)
+        St = _State32 if self.dtype == np.float32 else _State64
+        self.state = St()
+        for name in ("pos", "vel", "velLF", "acc", "presF", "measures", "sortedPos", "sortedVel", "hash",
+                     "index", "cellStart", "cellEnd"):
+            setattr(self.state, name, getattr(self, name).ctypes.data)
+        self.state.n = n
+        self.state.numCells = self.num_cells
+        self.state.elapsed = 0.0
+        self.state.time_shift = 0.0
+        self.state.time_relax = 1000 * float(np.float32(P.delta_time))
+        self.state.wave_height = 0.0
+
+    @property
+    def elapsed(self) -> float:
+        return float(self.state.elapsed)
+
+    def update(self, steps: int = 1) -> None:
+        fn = lib().oracle_update_f32 if self.dtype == np.float32 else lib().oracle_update_f64
+        for _ in range(steps):
+            fn(C.byref(self.P), C.byref(self.state))
+
+
+def dambreak_init(P, n: int) -> np.ndarray:
+    out = np.zeros((n, 4), np.float32)
+    lib().oracle_dambreak_init(C.byref(P), _ptr(out), C.c_uint32(n))
+    return out
+
+
+def poiseuille_init(P, n: int) -> np.ndarray:
+    out = np.zeros((n, 4), np.float32)
+    lib().oracle_poiseuille_init(C.byref(P), _ptr(out), C.c_uint32(n))
+    return out
+
+
+def peristalsis_init(P, n: int) -> np.ndarray:
+    out = np.zeros((n, 4), np.float32)
+    lib().oracle_peristalsis_init(C.byref(P), _ptr(out), C.c_uint32(n))
+    return out
+
+
+def peristalsis_mass_sum(P, pos4) -> float:
+    pos4 = _f(pos4, np.float32)
+    return float(lib().oracle_peristalsis_mass_sum(C.byref(P), _ptr(pos4), C.c_uint32(pos4.shape[0])))
+
+
+def dambreak_count(fluid, bo, grid) -> int:
+    return int(lib().oracle_dambreak_count(*[C.c_uint32(int(v)) for v in fluid], C.c_int32(bo),
+                                           *[C.c_uint32(int(v)) for v in grid]))
+
+
+def run_pipeline(P, pos, vel_lf, measures=None, dtype=np.float32):
+    """hash -> stable sort -> cellStart/End -> gather -> density, returning the
+    intermediates (all in the reference's sorted order)."""
+    pos32 = np.asarray(pos, np.float32)
+    h, idx = calc_hash(P, pos32)
+    sh, perm = sort_by_key(h, idx)
+    nc = int(P.grid_size[0]) * int(P.grid_size[1]) * int(P.grid_size[2])
+    cs, ce = find_cell_start(sh, nc)
+    sp, sv = gather(perm, np.asarray(pos, dtype), np.asarray(vel_lf, dtype), dtype=dtype)
+    meas = density(P, sp, cs, ce, measures_in=measures, dtype=dtype)
+    return dict(hash=h, sorted_hash=sh, index=perm, cell_start=cs, cell_end=ce, sorted_pos=sp,
+                sorted_vel=sv, measures=meas)

@@ -1,0 +1,259 @@
+"""GPU parity of every hot-path stage against the oracle, through the C ABI.
+
+Bars (each stage gets the oracle's own inputs, so errors do not compound):
+  hash, sort permutation, cellStart (all cells), cellEnd (non-empty cells),
+  the sorted gather and integrate: bit-exact.
+  density: 1e-5 relative per particle against the FP32 oracle; the FP64
+  oracle error is reported and bounded at 1e-5.
+  pressure (Tait): |dp| <= 1e-5 * max|p| + gamma*B*2e-6 (the EOS amplifies
+  density rounding by gamma/(rho/rho0)^gamma - 1 near rest).
+  forces: normwise max|da| / max|a| <= 1e-5 against the FP32 oracle.
+"""
+import ctypes as C
+import zlib
+
+import numpy as np
+import pytest
+
+from cmag_amd import _lib as L
+from cmag_amd import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _gpu():
+    from tests.gpu_util import Stages, normwise
+    return Stages, normwise
+
+
+def perturb(pos, vel_scale, frac, rng, dims=2):
+    pos = pos.copy()
+    n = pos.shape[0]
+    d = np.zeros((n, 3))
+    d[:, :dims] = rng.uniform(-frac, frac, (n, dims))
+    vel = np.zeros_like(pos)
+    vel[:, :dims] = rng.normal(scale=vel_scale, size=(n, dims))
+    return pos, vel, d
+
+
+def make_case(name):
+    """(params, positions, leapfrog velocities) of a small instance per system"""
+    from oracle import oracle as O
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if name == "ref_dambreak":
+        fluid, bo, grid, r = (48, 40, 1), 3, (256, 128, 4), 1 / 256
+        p = S.params_dambreak(fluid, bo, grid, r)
+        n = O.dambreak_count(fluid, bo, grid)
+        pos = O.dambreak_init(p, n)
+        fl = pos[:, 3] == 4
+        pos[fl, :2] += rng.uniform(-0.1, 0.1, (fl.sum(), 2)) * 2 * r
+        vel = np.zeros_like(pos)
+        vel[fl, :2] = rng.normal(scale=0.05, size=(fl.sum(), 2))
+    elif name == "ref_poiseuille":
+        r = 1.0 / (2 * 58 * 1000)
+        p = S.params_poiseuille((16, 58, 1), 3, (16, 64, 4), r)
+        n = 16 * 58 + 2 * 16 * 3
+        pos = O.poiseuille_init(p, n)
+        fl = pos[:, 3] == 0
+        pos[fl, :2] += rng.uniform(-0.05, 0.05, (fl.sum(), 2)) * 2 * r
+        vel = np.zeros_like(pos)
+        vel[fl, 0] = rng.normal(scale=1e-4, size=fl.sum())
+    elif name == "ref_peristalsis":
+        r = 1.0 / (2 * (64 - 6) * 1000)
+        p = S.params_peristalsis(1e-4, (64, 58, 1), 0.6 * 35 * r, 1e-2, 1e-4, (0, 0, 0), 3, (64, 128, 4), r)
+        n = 64 * 58 + 2 * 64 * 3
+        pos = O.peristalsis_init(p, n)
+        p.particle_mass = p.rest_density / O.peristalsis_mass_sum(p, pos)
+        pos = O.configure_boundary(p, pos, 0.5 * p.amplitude)
+        fl = pos[:, 3] == 0
+        pos[fl, :2] += rng.uniform(-0.05, 0.05, (fl.sum(), 2)) * 2 * r
+        vel = np.zeros_like(pos)
+        vel[fl, :2] = rng.normal(scale=1e-4, size=(fl.sum(), 2))
+    elif name == "cfg_channel":
+        sc = S.poiseuille_channel()
+        p, pos = sc.params, sc.pos.copy()
+        vel = np.zeros_like(pos)
+        vel[:sc.num_fluid, 0] = rng.normal(scale=1e-2, size=sc.num_fluid)
+    elif name in ("cfg_dam3d", "cfg_dam3d_morton"):
+        order = L.HASH_MORTON if name.endswith("morton") else L.HASH_LINEAR
+        sc = S.dambreak3d(dx=0.02, hash_order=order)
+        p, pos = sc.params, sc.pos.copy()
+        vel = np.zeros_like(pos)
+        vel[:sc.num_fluid, :3] = rng.normal(scale=0.1, size=(sc.num_fluid, 3))
+    else:
+        raise KeyError(name)
+    return p, pos.astype(np.float32), vel.astype(np.float32)
+
+
+CASES = ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_channel", "cfg_dam3d", "cfg_dam3d_morton"]
+
+
+# ---------------------------------------------------------------- hash
+@pytest.mark.parametrize("case", CASES)
+def test_hash_bitexact(device, oracle, case):
+    Stages, _ = _gpu()
+    p, pos, _ = make_case(case)
+    h, i = Stages.calc_hash(p, pos)
+    wh, wi = oracle.calc_hash(p, pos)
+    assert np.array_equal(h, wh) and np.array_equal(i, wi)
+
+
+@pytest.mark.parametrize("rule", [L.GRIDPOS_DIV_CELL, L.GRIDPOS_HALF_DIV_RADIUS])
+@pytest.mark.parametrize("order", [L.HASH_LINEAR, L.HASH_MORTON])
+def test_hash_edge_positions(device, oracle, rule, order):
+    """exact cell boundaries, one ulp below them, negative cells and cells past
+    the grid (which wrap)"""
+    Stages, _ = _gpu()
+    from tests.test_oracle import grid_params
+    p = grid_params(rule=rule, order=order, grid=(16, 8, 4))
+    rng = np.random.default_rng(12)
+    c = rng.integers(-20, 40, size=(4000, 3)).astype(np.float32)
+    base = np.array([-2.0, -1.0, -0.5], np.float32) + c * np.float32(0.25)
+    below = np.nextafter(base, np.float32(-100))
+    jitter = base + rng.uniform(0, 0.25, size=base.shape).astype(np.float32)
+    pts = np.concatenate([base, below, jitter]).astype(np.float32)
+    pos = np.zeros((pts.shape[0], 4), np.float32)
+    pos[:, :3] = pts
+    h, _ = Stages.calc_hash(p, pos)
+    wh, _ = oracle.calc_hash(p, pos)
+    assert np.array_equal(h, wh)
+
+
+# ---------------------------------------------------------------- sort
+SORT_CASES = [
+    (0, 8, "random"), (1, 8, "random"), (2, 1, "random"), (100, 14, "random"), (4095, 20, "random"),
+    (4096, 20, "dups"), (4097, 24, "sorted"), (65536 + 3, 24, "reversed"), (300_000, 20, "dups"),
+    (1_000_003, 24, "random"), (1 << 20, 32, "random"), (50_000, 16, "allsame"), (123_457, 9, "random"),
+]
+
+
+@pytest.mark.parametrize("n,bits,kind", SORT_CASES)
+def test_sort_is_stable_argsort(device, n, bits, kind):
+    Stages, _ = _gpu()
+    rng = np.random.default_rng(n + bits)
+    hi = 1 << bits
+    if kind == "random":
+        keys = rng.integers(0, hi, n, dtype=np.uint64).astype(np.uint32)
+    elif kind == "dups":
+        keys = rng.integers(0, min(hi, 37), n).astype(np.uint32)
+    elif kind == "sorted":
+        keys = np.sort(rng.integers(0, hi, n)).astype(np.uint32)
+    elif kind == "reversed":
+        keys = np.sort(rng.integers(0, hi, n))[::-1].astype(np.uint32)
+    else:
+        keys = np.full(n, hi - 1, np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = Stages.sort(keys, vals, bits)
+    perm = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, perm)
+    assert np.array_equal(k, keys[perm])
+
+
+# ---------------------------------------------------------------- reorder
+@pytest.mark.parametrize("case", ["ref_dambreak", "cfg_dam3d_morton", "cfg_channel"])
+def test_reorder_bitexact(device, oracle, case):
+    Stages, _ = _gpu()
+    p, pos, vel = make_case(case)
+    ref = oracle.run_pipeline(p, pos, vel)
+    nc = S.num_cells(p)
+    cs, ce, sp, sv = Stages.reorder(ref["sorted_hash"], ref["index"], pos, vel, nc)
+    assert np.array_equal(cs, ref["cell_start"])
+    occ = ref["cell_start"] != L.CELL_EMPTY
+    assert np.array_equal(ce[occ], ref["cell_end"][occ])
+    assert np.all(ce[~occ] == 0xDEADBEEF)  # empty cells' cellEnd untouched
+    assert np.array_equal(sp, ref["sorted_pos"]) and np.array_equal(sv, ref["sorted_vel"])
+
+
+# ---------------------------------------------------------------- density
+@pytest.mark.parametrize("case", CASES)
+def test_density_parity(device, oracle, case):
+    Stages, _ = _gpu()
+    p, pos, vel = make_case(case)
+    ref = oracle.run_pipeline(p, pos, vel)
+    sp, cs, ce = ref["sorted_pos"], ref["cell_start"], ref["cell_end"]
+    seed = np.zeros((pos.shape[0], 4), np.float32)
+    seed[:, 2], seed[:, 3] = 7.0, 9.0           # z, w must survive where the reference leaves them
+    m = Stages.density(p, sp, cs, ce, measures_in=seed)
+    want = oracle.density(p, sp, cs, ce, measures_in=seed)
+    rel = np.abs(m[:, 0] - want[:, 0]) / np.abs(want[:, 0])
+    assert rel.max() <= 1e-5, rel.max()
+    f64 = oracle.density(p, sp, cs, ce, measures_in=seed, dtype=np.float64)
+    rel64 = np.abs(m[:, 0] - f64[:, 0]) / np.abs(f64[:, 0])
+    assert rel64.max() <= 1e-5, rel64.max()
+    pmax = np.abs(want[:, 1]).max()
+    tol_p = 1e-5 * pmax + (float(p.gamma) * float(p.B) * 2e-6 if p.eos == L.EOS_TAIT else 0.0)
+    assert np.abs(m[:, 1] - want[:, 1]).max() <= tol_p
+    if p.system == L.SYS_PERISTALSIS:
+        assert np.array_equal(m[:, 2], np.zeros(len(m), np.float32)) and np.array_equal(m[:, 3], sp[:, 3])
+    else:
+        assert np.all(m[:, 2] == 7.0) and np.all(m[:, 3] == 9.0)
+
+
+# ---------------------------------------------------------------- forces
+@pytest.mark.parametrize("case", [c for c in CASES if c != "ref_peristalsis"])
+def test_force_parity(device, oracle, case):
+    Stages, normwise = _gpu()
+    p, pos, vel = make_case(case)
+    ref = oracle.run_pipeline(p, pos, vel)
+    args = (ref["measures"], ref["sorted_pos"], ref["sorted_vel"], ref["index"], ref["cell_start"],
+            ref["cell_end"])
+    seed = np.full((pos.shape[0], 4), 3.0, np.float32)
+    a = Stages.force(p, *args, acc_in=seed)
+    want = oracle.force(p, *args, acc_in=seed)
+    err = normwise(a[:, :3], want[:, :3])
+    assert err <= 1e-5, err
+    if p.system == L.SYS_DAMBREAK:
+        walls = pos[:, 3] != 4.0
+        assert np.all(a[walls] == 3.0)      # wall rows are never written
+    want64 = oracle.force(p, *args, acc_in=seed, dtype=np.float64)
+    assert normwise(a[:, :3], want64[:, :3]) <= 1e-4
+
+
+def test_peristalsis_pressure_and_viscous_parity(device, oracle):
+    Stages, normwise = _gpu()
+    p, pos, vel = make_case("ref_peristalsis")
+    ref = oracle.run_pipeline(p, pos, vel)
+    m, sp, sv, idx, cs, ce = (ref["measures"], ref["sorted_pos"], ref["sorted_vel"], ref["index"],
+                              ref["cell_start"], ref["cell_end"])
+    pf = Stages.pressure_force(p, m, sp, idx, cs, ce)
+    assert normwise(pf[:, :3], oracle.pressure_force(p, m, sp, idx, cs, ce)[:, :3]) <= 1e-5
+    vf = Stages.viscous_force(p, m, sp, sv, idx, cs, ce)
+    assert normwise(vf[:, :3], oracle.viscous_force(p, m, sp, sv, idx, cs, ce)[:, :3]) <= 1e-5
+
+
+# ---------------------------------------------------------------- integrate / walls
+@pytest.mark.parametrize("case", ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_dam3d"])
+def test_integrate_bitexact(device, oracle, case):
+    Stages, _ = _gpu()
+    p, pos, vel = make_case(case)
+    rng = np.random.default_rng(13)
+    acc = rng.normal(scale=50.0, size=pos.shape).astype(np.float32)
+    pres = rng.normal(scale=50.0, size=pos.shape).astype(np.float32)
+    vlf = rng.normal(size=pos.shape).astype(np.float32)
+    if p.system == L.SYS_PERISTALSIS:
+        p.is_boundary_configuration = 0
+    got = Stages.integrate(p, pos, vel, vlf, acc, pres, wave_time=0.0123)
+    want = oracle.integrate(p, pos, vel, vlf, acc, pres, wave_time=0.0123)
+    if p.system == L.SYS_PERISTALSIS:
+        fl = pos[:, 3] == 0
+        for g, w in zip(got, want):
+            assert np.array_equal(g[fl], w[fl])
+        # walls: sinf on device vs libm sinf on the host, a few ulps apart
+        assert np.allclose(got[0][~fl], want[0][~fl], rtol=0, atol=4e-6 * float(p.particle_radius))
+    else:
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+
+
+def test_boundary_kernels(device, oracle):
+    Stages, _ = _gpu()
+    p, pos, _ = make_case("ref_dambreak")
+    assert np.array_equal(Stages.pos_kernel("cmag_shift_right_boundary", p, pos), oracle.shift_right_boundary(p, pos))
+    assert np.array_equal(Stages.pos_kernel("cmag_remove_right_boundary", p, pos),
+                          oracle.remove_right_boundary(p, pos))
+    p, pos, _ = make_case("ref_peristalsis")
+    got = Stages.pos_kernel("cmag_configure_boundary", p, pos, C.c_float(0.3 * p.amplitude))
+    want = oracle.configure_boundary(p, pos, 0.3 * p.amplitude)
+    assert np.allclose(got, want, rtol=0, atol=4e-6 * float(p.particle_radius))

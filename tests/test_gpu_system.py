@@ -1,0 +1,160 @@
+"""System objects on the GPU against the oracle's update(): the reference
+classes through their Python mirror, HIP-graph replay, and the state API."""
+import numpy as np
+import pytest
+
+from cmag_amd import _lib as L
+from cmag_amd import scenarios as S
+from cmag_amd.systems import DamBreakSystem, ParamSystem, PeristalsisSystem, PoiseuilleFlowSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_after(oracle, sysobj_params, pos, steps, vel=None, measures=None):
+    sim = oracle.Simulation(sysobj_params, pos, vel, vel, measures)
+    sim.update(steps)
+    return sim
+
+
+def check_grid_state(sysobj, sim):
+    """hash, index, cellStart bit-exact; cellEnd on occupied cells"""
+    assert np.array_equal(sysobj.getUintArray(L.ARRAY_HASH), sim.hash)
+    assert np.array_equal(sysobj.getUintArray(L.ARRAY_INDEX), sim.index)
+    cs = sysobj.getUintArray(L.ARRAY_CELLSTART)
+    assert np.array_equal(cs, sim.cellStart)
+    occ = cs != L.CELL_EMPTY
+    assert np.array_equal(sysobj.getUintArray(L.ARRAY_CELLEND)[occ], sim.cellEnd[occ])
+
+
+def rel_field(got, want, cols=3):
+    scale = np.abs(want[:, :cols]).max()
+    return float(np.abs(got[:, :cols].astype(np.float64) - want[:, :cols]).max() / max(scale, 1e-30))
+
+
+def test_dambreak_update_matches_oracle(device, oracle):
+    fluid, bo, grid, r = (64, 64, 1), 1, (256, 128, 4), 1 / 256
+    s = DamBreakSystem(fluid, bo, grid, r, False)
+    s.reset()
+    p = s.params
+    pos0 = s.getArray(DamBreakSystem.POSITION)
+    assert np.array_equal(pos0, oracle.dambreak_init(p, s.getNumParticles()))
+    s.update()
+    sim = oracle_after(oracle, p, pos0, 1)
+    check_grid_state(s, sim)
+    assert np.allclose(s.getArray(DamBreakSystem.MEASURES)[:, 0], sim.measures[:, 0], rtol=1e-5)
+    assert rel_field(s.getArray(DamBreakSystem.ACCELERATION), sim.acc) <= 1e-5
+    assert rel_field(s.getArray(DamBreakSystem.VELOCITY), sim.vel) <= 1e-5
+    for _ in range(19):
+        s.update()
+    sim.update(19)
+    check_grid_state(s, sim)
+    assert rel_field(s.getArray(DamBreakSystem.POSITION), sim.pos) <= 1e-6
+    assert rel_field(s.getArray(DamBreakSystem.VELOCITY), sim.vel) <= 1e-3
+    assert s.getElapsedTime() == pytest.approx(20 * 1e-4, rel=1e-5)
+
+
+def test_poiseuille_report_run_matches_oracle(device, oracle):
+    """The Poiseuille.Report configuration (dump.h:17-27), 225 steps (t = 0.0225)."""
+    r = 1.0 / (2 * (64 - 2 * 3) * 1000)
+    s = PoiseuilleFlowSystem((16, 58, 1), 3, (16, 64, 4), r, False)
+    s.reset()
+    p = s.params
+    pos0 = s.getArray(0)
+    sim = oracle_after(oracle, p, pos0, 0)
+    steps = 0
+    while s.getElapsedTime() < 0.0225:
+        s.update()
+        steps += 1
+    sim.update(steps)
+    check_grid_state(s, sim)
+    v = s.getArray(PoiseuilleFlowSystem.VELOCITY)
+    assert rel_field(v, sim.vel, cols=1) <= 1e-4
+    assert rel_field(s.getArray(0), sim.pos) <= 1e-6
+
+
+def test_peristalsis_update_matches_oracle(device, oracle):
+    r = 1.0 / (2 * (64 - 6) * 1000)
+    s = PeristalsisSystem(1e-4, (64, 58, 1), 0.6 * 35 * r, 100 * 1e-4, 1e-4, (0, 0, 0), 3, (64, 128, 4), r, False)
+    s.Reset()
+    p = s.params
+    assert p.particle_mass > 0
+    pos0 = s.getArray(PeristalsisSystem.POSITION)
+    sim = oracle.Simulation(p, pos0, None, None, np.zeros_like(pos0))
+    for _ in range(30):
+        s.Update()
+    sim.update(30)
+    check_grid_state(s, sim)
+    assert rel_field(s.getArray(PeristalsisSystem.POSITION), sim.pos) <= 1e-6
+    assert rel_field(s.getArray(PeristalsisSystem.VISCOUSFORCE), sim.acc) <= 1e-4
+    assert rel_field(s.getArray(PeristalsisSystem.PRESSUREFORCE), sim.presF) <= 1e-4
+    assert s.GetElapsedTime() == pytest.approx(sim.elapsed, rel=1e-6)
+
+
+@pytest.mark.parametrize("order", [L.HASH_LINEAR, L.HASH_MORTON])
+def test_dam3d_config_system_matches_oracle(device, oracle, order):
+    sc = S.dambreak3d(dx=0.02, hash_order=order)
+    s = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    sim = oracle.Simulation(sc.params, sc.pos)
+    s.update()
+    sim.update(1)
+    check_grid_state(s, sim)
+    assert rel_field(s.getArray(L.ARRAY_ACCELERATION)[:sc.num_fluid], sim.acc[:sc.num_fluid]) <= 1e-5
+    for _ in range(9):
+        s.update()
+    sim.update(9)
+    check_grid_state(s, sim)
+    assert rel_field(s.getArray(L.ARRAY_POSITION), sim.pos) <= 1e-6
+
+
+def test_graph_replay_equals_eager(device):
+    sc = S.poiseuille_channel()
+    a = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    b = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    a.run(25, use_graph=False)
+    b.run(25, use_graph=True)
+    for arr in (L.ARRAY_POSITION, L.ARRAY_VELOCITY, L.ARRAY_VELOCITYLEAPFROG):
+        assert np.array_equal(a.getArray(arr), b.getArray(arr))
+    assert a.getElapsedTime() == b.getElapsedTime()
+    # eager work between replays, then more replays
+    b.run(5, use_graph=True)
+    b.update()
+    b.run(5, use_graph=True)
+    a.run(11, use_graph=False)
+    assert np.array_equal(a.getArray(L.ARRAY_POSITION), b.getArray(L.ARRAY_POSITION))
+
+
+def test_set_get_array_roundtrip_and_errors(device):
+    s = DamBreakSystem((16, 16, 1), 1, (64, 32, 4), 1 / 64, False)
+    s.reset()
+    n = s.getNumParticles()
+    data = np.arange(n * 4, dtype=np.float32).reshape(n, 4)
+    s.setArray(DamBreakSystem.VELOCITY, data)
+    assert np.array_equal(s.getArray(DamBreakSystem.VELOCITY), data)
+    s.setArray(DamBreakSystem.VELOCITY, data[:5] * 2, start=10, count=5)
+    assert np.array_equal(s.getArray(DamBreakSystem.VELOCITY, 10, 5), data[:5] * 2)
+    with pytest.raises(L.CmagError):
+        s.setArray(DamBreakSystem.VELOCITY, data, start=1)
+    with pytest.raises(L.CmagError):
+        s.getArray(99)
+    with pytest.raises(L.CmagError):
+        ParamSystem(sc_bad_params(), 10)
+
+
+def sc_bad_params():
+    p = S.poiseuille_channel().params.copy()
+    p.grid_size[1] = 100
+    return p
+
+
+def test_right_boundary_controls_on_device(device, oracle):
+    s = DamBreakSystem((32, 32, 1), 1, (128, 64, 4), 1 / 64, False)
+    s.reset()
+    p = s.params
+    pos = s.getArray(0)
+    s.changeRightBoundary()
+    assert np.array_equal(s.getArray(0), oracle.shift_right_boundary(p, pos))
+    assert s.params.right_boundary == pytest.approx(p.right_boundary + 32 * 2 / 64)
+    s.update()
+    s.removeRightBoundary()
+    assert s.getElapsedTime() == 0.0
+    assert s.params.right_boundary == np.float32(0xFFFFFFFF)
