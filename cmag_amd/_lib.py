@@ -103,6 +103,7 @@ ARRAY_INDEX = 17
 ARRAY_CELLSTART = 18
 ARRAY_CELLEND = 19
 CELL_EMPTY = 0xFFFFFFFF
+STAGES = ("hash", "sort", "reorder", "density", "force", "integrate")
 
 _P = C.POINTER(CmagParams)
 _vp = C.c_void_p
@@ -157,6 +158,8 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_system_set_params": (C.c_int, [_vp, _P]),
     "cmag_system_set_stream": (C.c_int, [_vp, _vp]),
     "cmag_system_synchronize": (C.c_int, [_vp]),
+    "cmag_system_enable_timing": (C.c_int, [_vp, C.c_int]),
+    "cmag_system_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]),
     "cmag_system_change_right_boundary": (C.c_int, [_vp]),
     "cmag_system_remove_right_boundary": (C.c_int, [_vp]),
     "cmag_system_change_gravity": (C.c_int, [_vp]),

@@ -87,9 +87,11 @@ __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cel
         }
     }
     if (i == n - 1 && h < numCells) cellEnd[h] = n;
-    uint32_t k = index[i];
-    sortedPos[i] = oldPos[k];
-    sortedVel[i] = oldVel[k];
+    const uint32_t k = index[i];
+    if (k < n) {  // a valid permutation always passes; anything else is skipped
+        sortedPos[i] = oldPos[k];
+        sortedVel[i] = oldVel[k];
+    }
 }
 
 int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,

@@ -43,7 +43,7 @@ __device__ __forceinline__ float eos(const cmag_params& P, float dens) {
 template <int DIM, typename F>
 __device__ __forceinline__ void for_each_cell(const cmag_params& P, const Cell& gp,
                                               const uint32_t* __restrict__ cellStart,
-                                              const uint32_t* __restrict__ cellEnd, F&& f) {
+                                              const uint32_t* __restrict__ cellEnd, uint32_t n, F&& f) {
     const int cc = P.cellcount;
     const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
     for (int dz = zlo; dz <= zhi; ++dz) {
@@ -58,7 +58,9 @@ __device__ __forceinline__ void for_each_cell(const cmag_params& P, const Cell& 
                 const uint32_t h = grid_hash(P, nx, ny, nz);
                 const uint32_t start = cellStart[h];
                 if (start == CMAG_CELL_EMPTY) continue;
-                f(start, cellEnd[h], swx, swy, swz);
+                // ranges are clamped to the particle count: consistent cell
+                // arrays never need it, inconsistent ones cannot read past n
+                f(start, min(cellEnd[h], n), swx, swy, swz);
             }
         }
     }
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(256) void density_kernel(const cmag_params P, float
     const float inv_h = 1.0f / h;
     const float r2max = 4.0f * h * h;
     float sum = 0.0f;
-    for_each_cell<DIM>(P, gp, cellStart, cellEnd,
+    for_each_cell<DIM>(P, gp, cellStart, cellEnd, n,
                        [&](uint32_t start, uint32_t end, float swx, float swy, float swz) {
                            float part = 0.0f;
                            for (uint32_t j = start; j < end; ++j) {
@@ -149,7 +151,7 @@ __global__ __launch_bounds__(256) void force_kernel(const cmag_params P, float4*
     const float av_eps = 0.001f * h * h;
     const float visc_k = m * (P.mu + P.mu);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    for_each_cell<DIM>(P, gp, cellStart, cellEnd, [&](uint32_t start, uint32_t end, float swx,
+    for_each_cell<DIM>(P, gp, cellStart, cellEnd, n, [&](uint32_t start, uint32_t end, float swx,
                                                       float swy, float swz) {
         float tx = 0.0f, ty = 0.0f, tz = 0.0f;
         for (uint32_t j = start; j < end; ++j) {
@@ -196,7 +198,8 @@ __global__ __launch_bounds__(256) void force_kernel(const cmag_params P, float4*
         }
         fx += tx; fy += ty; fz += tz;
     });
-    acc[index[i]] = make_float4(fx, fy, fz, 0.0f);
+    const uint32_t k = index[i];
+    if (k < n) acc[k] = make_float4(fx, fy, fz, 0.0f);
 }
 
 // Peristalsis: pressure and viscous passes; the fused variant produces both
@@ -223,7 +226,7 @@ __global__ __launch_bounds__(256) void peristalsis_force_kernel(
     const float gc = P.grad_coeff;
     const float visc_k = m * (P.mu + P.mu);
     float vx = 0.f, vy = 0.f, vz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-    for_each_cell<DIM>(P, gp, cellStart, cellEnd, [&](uint32_t start, uint32_t end, float swx,
+    for_each_cell<DIM>(P, gp, cellStart, cellEnd, n, [&](uint32_t start, uint32_t end, float swx,
                                                       float swy, float swz) {
         float tvx = 0.f, tvy = 0.f, tvz = 0.f, tpx = 0.f, tpy = 0.f, tpz = 0.f;
         for (uint32_t j = start; j < end; ++j) {
@@ -256,6 +259,7 @@ __global__ __launch_bounds__(256) void peristalsis_force_kernel(
         px += tpx; py += tpy; pz += tpz;
     });
     const uint32_t k = index[i];
+    if (k >= n) return;
     if (WANT_VISC) viscF[k] = make_float4(vx, vy, vz, 0.0f);
     if (WANT_PRES) presF[k] = make_float4(px, py, pz, 0.0f);
 }

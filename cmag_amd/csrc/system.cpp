@@ -47,8 +47,36 @@ struct cmag_system {
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
 
+    // per-stage event timing: ev[k] recorded before stage k, ev[NUM] after the last
+    bool timing = false;
+    hipEvent_t ev[CMAG_NUM_STAGES + 1] = {};
+    double stage_ms[CMAG_NUM_STAGES] = {};
+    uint32_t stage_count[CMAG_NUM_STAGES] = {};
+    bool pending = false;
+
+    int mark(int k) {
+        if (timing) CMAG_HIP_TRY(hipEventRecord(ev[k], stream));
+        return CMAG_OK;
+    }
+
+    // fold the previous step's events into the totals (waits for that step)
+    int collect() {
+        if (!timing || !pending) return CMAG_OK;
+        CMAG_HIP_TRY(hipEventSynchronize(ev[CMAG_NUM_STAGES]));
+        for (int k = 0; k < CMAG_NUM_STAGES; ++k) {
+            float ms = 0.0f;
+            CMAG_HIP_TRY(hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+            stage_ms[k] += ms;
+            stage_count[k] += 1;
+        }
+        pending = false;
+        return CMAG_OK;
+    }
+
     ~cmag_system() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, sortWs};
         for (void* b : bufs)
@@ -91,7 +119,9 @@ struct cmag_system {
         const bool odd = passes() % 2 == 1;
         uint32_t* hk = odd ? hashAlt : hash;
         uint32_t* hv = odd ? indexAlt : index;
+        CMAG_TRY(mark(CMAG_STAGE_HASH));
         CMAG_TRY(launch_calc_hash(P, pos, hk, hv, n, stream));
+        CMAG_TRY(mark(CMAG_STAGE_SORT));
         bool in_alt = false;
         CMAG_TRY(radix_sort_pairs(hk, hv, odd ? hash : hashAlt, odd ? index : indexAlt, n, key_bits,
                                   sortWs, sortWsBytes, &in_alt, stream));
@@ -103,9 +133,12 @@ struct cmag_system {
             CMAG_TRY(launch_copy_u32(hash, hk, n, stream));
             CMAG_TRY(launch_copy_u32(index, hv, n, stream));
         }
+        CMAG_TRY(mark(CMAG_STAGE_REORDER));
         CMAG_TRY(launch_reorder(cellStart, cellEnd, sortedPos, sortedVel, hash, index, pos, velLF, n,
                                 numCells, stream));
+        CMAG_TRY(mark(CMAG_STAGE_DENSITY));
         CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream));
+        CMAG_TRY(mark(CMAG_STAGE_FORCE));
         if (P.system == CMAG_SYS_PERISTALSIS) {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
                                                cellStart, cellEnd, n, stream));
@@ -113,7 +146,11 @@ struct cmag_system {
             CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
                                   stream));
         }
-        return launch_integrate(P, pos, vel, velLF, acc, presF, wave_time, n, stream);
+        CMAG_TRY(mark(CMAG_STAGE_INTEGRATE));
+        CMAG_TRY(launch_integrate(P, pos, vel, velLF, acc, presF, wave_time, n, stream));
+        CMAG_TRY(mark(CMAG_NUM_STAGES));
+        if (timing) pending = true;
+        return CMAG_OK;
     }
 
     // PeristalsisSystem::Update's host state machine, PeristalsisSystem.cpp:330-343;
@@ -136,6 +173,7 @@ struct cmag_system {
     }
 
     int update() {
+        CMAG_TRY(collect());
         float t_w = 0.0f;
         CMAG_TRY(peristalsis_prologue(&t_w));
         CMAG_TRY(enqueue_step(t_w));
@@ -153,8 +191,11 @@ struct cmag_system {
         invalidate_graph();
         if (stream == nullptr) return fail(CMAG_E_STATE, "graph capture needs a non-default stream");
         hipGraph_t g = nullptr;
+        const bool was_timing = timing;
+        timing = false;  // no event nodes inside the graph
         CMAG_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_step(0.0f);
+        timing = was_timing;
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc != CMAG_OK) {
             if (g) (void)hipGraphDestroy(g);
@@ -449,6 +490,27 @@ int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream) {
 int cmag_system_synchronize(cmag_system* s) {
     if (!s) return fail(CMAG_E_INVALID, "null system");
     CMAG_HIP_TRY(hipStreamSynchronize(s->stream));
+    return CMAG_OK;
+}
+
+int cmag_system_enable_timing(cmag_system* s, int on) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (on && !s->ev[0])
+        for (hipEvent_t& e : s->ev) CMAG_HIP_TRY(hipEventCreate(&e));
+    CMAG_TRY(s->collect());
+    s->timing = on != 0;
+    return CMAG_OK;
+}
+
+int cmag_system_stage_times(cmag_system* s, double* ms, uint32_t* counts, int max_stages) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_TRY(s->collect());
+    for (int k = 0; k < CMAG_NUM_STAGES && k < max_stages; ++k) {
+        if (ms) ms[k] = s->stage_ms[k];
+        if (counts) counts[k] = s->stage_count[k];
+        s->stage_ms[k] = 0.0;
+        s->stage_count[k] = 0;
+    }
     return CMAG_OK;
 }
 

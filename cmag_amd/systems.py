@@ -115,6 +115,16 @@ class SystemBase:
     def set_params(self, p: CmagParams) -> None:
         call("cmag_system_set_params", self._h, C.byref(p))
 
+    def enable_timing(self, on: bool = True) -> None:
+        call("cmag_system_enable_timing", self._h, 1 if on else 0)
+
+    def stage_times(self) -> dict:
+        """{stage: (total ms, launches)} since the last call; clears the totals"""
+        ms = (C.c_double * 6)()
+        cnt = (C.c_uint32 * 6)()
+        call("cmag_system_stage_times", self._h, ms, cnt, 6)
+        return {name: (ms[k], cnt[k]) for k, name in enumerate(L.STAGES)}
+
     def synchronize(self) -> None:
         call("cmag_system_synchronize", self._h)
 

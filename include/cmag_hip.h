@@ -206,6 +206,18 @@ int cmag_system_set_params(cmag_system* s, const cmag_params* p);
 int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream);
 int cmag_system_synchronize(cmag_system* s);
 
+/* stage timing (no reference counterpart): with timing on, update() records
+ * HIP events between its stages on the object's stream; stage_times returns
+ * the accumulated milliseconds and launch counts of each stage, in the order
+ * hash, sort, reorder, density, force, integrate, and clears them. */
+enum cmag_stage {
+    CMAG_STAGE_HASH = 0, CMAG_STAGE_SORT = 1, CMAG_STAGE_REORDER = 2,
+    CMAG_STAGE_DENSITY = 3, CMAG_STAGE_FORCE = 4, CMAG_STAGE_INTEGRATE = 5,
+    CMAG_NUM_STAGES = 6
+};
+int cmag_system_enable_timing(cmag_system* s, int on);
+int cmag_system_stage_times(cmag_system* s, double* ms, uint32_t* counts, int max_stages);
+
 /* scenario controls: fluidSystem.cpp:222-255, poiseuilleFlowSystem.cpp:207-210 */
 int cmag_system_change_right_boundary(cmag_system* s);
 int cmag_system_remove_right_boundary(cmag_system* s);

@@ -96,56 +96,54 @@ def _sfx(dtype) -> str:
 
 
 def gather(index, pos, vel, dtype=np.float32):
-    pos = _f(pos, dtype); vel = _f(vel, dtype)
+    # every converted array is bound to a name for the duration of the call:
+    # a temporary's buffer would be freed before the C code reads it
+    pos = _f(pos, dtype); vel = _f(vel, dtype); idx = _f(index, np.uint32)
     n = pos.shape[0]
     sp = np.empty_like(pos); sv = np.empty_like(vel)
-    getattr(lib(), "oracle_gather" + _sfx(dtype))(_ptr(sp), _ptr(sv), _ptr(_f(index, np.uint32)),
-                                                 _ptr(pos), _ptr(vel), C.c_uint32(n))
+    getattr(lib(), "oracle_gather" + _sfx(dtype))(_ptr(sp), _ptr(sv), _ptr(idx), _ptr(pos), _ptr(vel),
+                                                 C.c_uint32(n))
     return sp, sv
 
 
 def density(P, sorted_pos, cell_start, cell_end, measures_in=None, dtype=np.float32):
     sp = _f(sorted_pos, dtype)
+    cs, ce = _f(cell_start, np.uint32), _f(cell_end, np.uint32)
     n = sp.shape[0]
     meas = np.zeros((n, 4), dtype) if measures_in is None else np.array(measures_in, dtype=dtype, copy=True)
-    getattr(lib(), "oracle_density" + _sfx(dtype))(C.byref(P), _ptr(meas), _ptr(sp),
-                                                  _ptr(_f(cell_start, np.uint32)),
-                                                  _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    getattr(lib(), "oracle_density" + _sfx(dtype))(C.byref(P), _ptr(meas), _ptr(sp), _ptr(cs), _ptr(ce),
+                                                  C.c_uint32(n))
     return meas
 
 
 def force(P, measures, sorted_pos, sorted_vel, index, cell_start, cell_end, acc_in=None,
           dtype=np.float32):
-    sp = _f(sorted_pos, dtype)
+    sp, m, sv = _f(sorted_pos, dtype), _f(measures, dtype), _f(sorted_vel, dtype)
+    idx, cs, ce = _f(index, np.uint32), _f(cell_start, np.uint32), _f(cell_end, np.uint32)
     n = sp.shape[0]
     acc = np.zeros((n, 4), dtype) if acc_in is None else np.array(acc_in, dtype=dtype, copy=True)
-    getattr(lib(), "oracle_force" + _sfx(dtype))(C.byref(P), _ptr(acc), _ptr(_f(measures, dtype)), _ptr(sp),
-                                                _ptr(_f(sorted_vel, dtype)), _ptr(_f(index, np.uint32)),
-                                                _ptr(_f(cell_start, np.uint32)), _ptr(_f(cell_end, np.uint32)),
-                                                C.c_uint32(n))
+    getattr(lib(), "oracle_force" + _sfx(dtype))(C.byref(P), _ptr(acc), _ptr(m), _ptr(sp), _ptr(sv), _ptr(idx),
+                                                _ptr(cs), _ptr(ce), C.c_uint32(n))
     return acc
 
 
 def pressure_force(P, measures, sorted_pos, index, cell_start, cell_end, dtype=np.float32):
-    sp = _f(sorted_pos, dtype)
+    sp, m = _f(sorted_pos, dtype), _f(measures, dtype)
+    idx, cs, ce = _f(index, np.uint32), _f(cell_start, np.uint32), _f(cell_end, np.uint32)
     n = sp.shape[0]
     out = np.zeros((n, 4), dtype)
-    getattr(lib(), "oracle_pressure_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(_f(measures, dtype)),
-                                                         _ptr(sp), _ptr(_f(index, np.uint32)),
-                                                         _ptr(_f(cell_start, np.uint32)),
-                                                         _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    getattr(lib(), "oracle_pressure_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(m), _ptr(sp), _ptr(idx),
+                                                         _ptr(cs), _ptr(ce), C.c_uint32(n))
     return out
 
 
 def viscous_force(P, measures, sorted_pos, sorted_vel, index, cell_start, cell_end, dtype=np.float32):
-    sp = _f(sorted_pos, dtype)
+    sp, m, sv = _f(sorted_pos, dtype), _f(measures, dtype), _f(sorted_vel, dtype)
+    idx, cs, ce = _f(index, np.uint32), _f(cell_start, np.uint32), _f(cell_end, np.uint32)
     n = sp.shape[0]
     out = np.zeros((n, 4), dtype)
-    getattr(lib(), "oracle_viscous_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(_f(measures, dtype)),
-                                                        _ptr(sp), _ptr(_f(sorted_vel, dtype)),
-                                                        _ptr(_f(index, np.uint32)),
-                                                        _ptr(_f(cell_start, np.uint32)),
-                                                        _ptr(_f(cell_end, np.uint32)), C.c_uint32(n))
+    getattr(lib(), "oracle_viscous_force" + _sfx(dtype))(C.byref(P), _ptr(out), _ptr(m), _ptr(sp), _ptr(sv),
+                                                        _ptr(idx), _ptr(cs), _ptr(ce), C.c_uint32(n))
     return out
 
 

@@ -45,7 +45,11 @@ def ptr(t: torch.Tensor) -> int:
 
 
 class Stages:
-    """C-ABI launchers over torch buffers (default stream, synchronised)."""
+    """C-ABI launchers over torch buffers (default stream, synchronised).
+
+    Every device buffer is bound to a local name until the launch has been
+    synchronised: a temporary's data_ptr would go back to the caching
+    allocator at once and the next allocation could reuse its memory."""
 
     @staticmethod
     def calc_hash(p, pos4: np.ndarray):
@@ -62,52 +66,69 @@ class Stages:
         nbytes = L.lib().cmag_sort_scratch_bytes(n, key_bits)
         scratch = torch.empty(max(nbytes, 1), dtype=torch.uint8, device="cuda")
         L.call("cmag_sort_by_key", ptr(k), ptr(v), n, key_bits, ptr(scratch), nbytes, None)
-        return to_np(k)[:n], to_np(v)[:n]
+        out = to_np(k)[:n], to_np(v)[:n]
+        del scratch
+        return out
 
     @staticmethod
     def reorder(sorted_hash, index, pos, vel, num_cells, cell_end_fill=0xDEADBEEF):
         n = sorted_hash.size
         cs, ce = empty_u32(num_cells), empty_u32(num_cells, cell_end_fill)
         sp, sv = empty_f4(n), empty_f4(n)
-        L.call("cmag_reorder_find_cell_start", ptr(cs), ptr(ce), ptr(sp), ptr(sv),
-               ptr(to_dev(sorted_hash.astype(np.uint32))), ptr(to_dev(index.astype(np.uint32))),
-               ptr(to_dev(pos.astype(np.float32))), ptr(to_dev(vel.astype(np.float32))), n, num_cells, None)
-        return to_np(cs)[:num_cells], to_np(ce)[:num_cells], to_np(sp)[:n], to_np(sv)[:n]
+        dh, di = to_dev(sorted_hash.astype(np.uint32)), to_dev(index.astype(np.uint32))
+        dp, dv = to_dev(pos.astype(np.float32)), to_dev(vel.astype(np.float32))
+        L.call("cmag_reorder_find_cell_start", ptr(cs), ptr(ce), ptr(sp), ptr(sv), ptr(dh), ptr(di),
+               ptr(dp), ptr(dv), n, num_cells, None)
+        out = to_np(cs)[:num_cells], to_np(ce)[:num_cells], to_np(sp)[:n], to_np(sv)[:n]
+        del dh, di, dp, dv
+        return out
 
     @staticmethod
     def density(p, sorted_pos, cs, ce, measures_in=None):
         n = sorted_pos.shape[0]
         m = to_dev(measures_in.astype(np.float32)) if measures_in is not None else empty_f4(n)
-        L.call("cmag_density", P(p), ptr(m), ptr(to_dev(sorted_pos.astype(np.float32))), ptr(to_dev(cs)),
-               ptr(to_dev(ce)), n, None)
-        return to_np(m)[:n]
+        dsp, dcs, dce = to_dev(sorted_pos.astype(np.float32)), to_dev(cs), to_dev(ce)
+        L.call("cmag_density", P(p), ptr(m), ptr(dsp), ptr(dcs), ptr(dce), n, None)
+        out = to_np(m)[:n]
+        del dsp, dcs, dce
+        return out
 
     @staticmethod
     def force(p, measures, sorted_pos, sorted_vel, index, cs, ce, acc_in=None):
         n = sorted_pos.shape[0]
         a = to_dev(acc_in.astype(np.float32)) if acc_in is not None else empty_f4(n)
-        L.call("cmag_force", P(p), ptr(a), ptr(to_dev(measures.astype(np.float32))),
-               ptr(to_dev(sorted_pos.astype(np.float32))), ptr(to_dev(sorted_vel.astype(np.float32))),
-               ptr(to_dev(index.astype(np.uint32))), ptr(to_dev(cs)), ptr(to_dev(ce)), n, None)
-        return to_np(a)[:n]
+        dm, dsp = to_dev(measures.astype(np.float32)), to_dev(sorted_pos.astype(np.float32))
+        dsv, di = to_dev(sorted_vel.astype(np.float32)), to_dev(index.astype(np.uint32))
+        dcs, dce = to_dev(cs), to_dev(ce)
+        L.call("cmag_force", P(p), ptr(a), ptr(dm), ptr(dsp), ptr(dsv), ptr(di), ptr(dcs), ptr(dce), n, None)
+        out = to_np(a)[:n]
+        del dm, dsp, dsv, di, dcs, dce
+        return out
 
     @staticmethod
     def pressure_force(p, measures, sorted_pos, index, cs, ce):
         n = sorted_pos.shape[0]
         a = empty_f4(n)
-        L.call("cmag_pressure_force", P(p), ptr(a), ptr(to_dev(measures.astype(np.float32))),
-               ptr(to_dev(sorted_pos.astype(np.float32))), ptr(to_dev(index.astype(np.uint32))),
-               ptr(to_dev(cs)), ptr(to_dev(ce)), n, None)
-        return to_np(a)[:n]
+        dm, dsp, di = to_dev(measures.astype(np.float32)), to_dev(sorted_pos.astype(np.float32)), \
+            to_dev(index.astype(np.uint32))
+        dcs, dce = to_dev(cs), to_dev(ce)
+        L.call("cmag_pressure_force", P(p), ptr(a), ptr(dm), ptr(dsp), ptr(di), ptr(dcs), ptr(dce), n, None)
+        out = to_np(a)[:n]
+        del dm, dsp, di, dcs, dce
+        return out
 
     @staticmethod
     def viscous_force(p, measures, sorted_pos, sorted_vel, index, cs, ce):
         n = sorted_pos.shape[0]
         a = empty_f4(n)
-        L.call("cmag_viscous_force", P(p), ptr(a), ptr(to_dev(measures.astype(np.float32))),
-               ptr(to_dev(sorted_pos.astype(np.float32))), ptr(to_dev(sorted_vel.astype(np.float32))),
-               ptr(to_dev(index.astype(np.uint32))), ptr(to_dev(cs)), ptr(to_dev(ce)), n, None)
-        return to_np(a)[:n]
+        dm, dsp = to_dev(measures.astype(np.float32)), to_dev(sorted_pos.astype(np.float32))
+        dsv, di = to_dev(sorted_vel.astype(np.float32)), to_dev(index.astype(np.uint32))
+        dcs, dce = to_dev(cs), to_dev(ce)
+        L.call("cmag_viscous_force", P(p), ptr(a), ptr(dm), ptr(dsp), ptr(dsv), ptr(di), ptr(dcs), ptr(dce),
+               n, None)
+        out = to_np(a)[:n]
+        del dm, dsp, dsv, di, dcs, dce
+        return out
 
     @staticmethod
     def integrate(p, pos, vel, vlf, acc, pres=None, wave_time=0.0):
@@ -117,7 +138,9 @@ class Stages:
         dpr = to_dev(pres.astype(np.float32)) if pres is not None else None
         L.call("cmag_integrate", P(p), ptr(dp), ptr(dv), ptr(dl), ptr(da), ptr(dpr) if dpr is not None else None,
                C.c_float(wave_time), n, None)
-        return to_np(dp)[:n], to_np(dv)[:n], to_np(dl)[:n]
+        out = to_np(dp)[:n], to_np(dv)[:n], to_np(dl)[:n]
+        del da, dpr
+        return out
 
     @staticmethod
     def pos_kernel(name, p, pos, *extra):

@@ -80,13 +80,19 @@ def test_peristalsis_update_matches_oracle(device, oracle):
     assert p.particle_mass > 0
     pos0 = s.getArray(PeristalsisSystem.POSITION)
     sim = oracle.Simulation(p, pos0, None, None, np.zeros_like(pos0))
-    for _ in range(30):
+    s.Update()
+    sim.update(1)
+    check_grid_state(s, sim)
+    assert rel_field(s.getArray(PeristalsisSystem.VISCOUSFORCE), sim.acc) <= 1e-5
+    assert rel_field(s.getArray(PeristalsisSystem.PRESSUREFORCE), sim.presF) <= 1e-5
+    # trajectories: forces built from velocity differences amplify rounding as
+    # steps accumulate, positions and velocities stay close
+    for _ in range(29):
         s.Update()
-    sim.update(30)
+    sim.update(29)
     check_grid_state(s, sim)
     assert rel_field(s.getArray(PeristalsisSystem.POSITION), sim.pos) <= 1e-6
-    assert rel_field(s.getArray(PeristalsisSystem.VISCOUSFORCE), sim.acc) <= 1e-4
-    assert rel_field(s.getArray(PeristalsisSystem.PRESSUREFORCE), sim.presF) <= 1e-4
+    assert rel_field(s.getArray(PeristalsisSystem.VELOCITY), sim.vel) <= 1e-3
     assert s.GetElapsedTime() == pytest.approx(sim.elapsed, rel=1e-6)
 
 
