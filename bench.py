@@ -2,9 +2,10 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--dx D] [--hash morton|linear]
 
-Workload: the 3-D dam break at 16,483,216 fluid particles (0.4 x 0.8 x 0.4
+Workload: the 3-D dam break at 16,484,816 fluid particles (0.4 x 0.8 x 0.4
 column, dx = 0.4/202, in a 1.6 x 1.0 x 0.4 tank with three wall layers on the
-floor and four sides, about 3.5M wall particles), Z-order cell hash, cell = 2h,
+floor and four sides, 3,586,416 wall particles), Z-order cell hash over 16x1x1
+cell bricks, cell = 2h,
 FP32, synthetic initial state.  One step = one update(): hash, stable radix
 sort, reorder + cellStart, density (Tait), fused pressure + artificial
 viscosity + Lennard-Jones wall force, integrate with clamps.  Inputs are
@@ -92,6 +93,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.4 / 202)
     ap.add_argument("--hash", choices=["morton", "linear"], default="morton")
+    ap.add_argument("--morton-xbits", type=int, default=4,
+                    help="Z order over bricks of 2^xbits cells along x (0 = plain Morton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -107,7 +110,9 @@ def main() -> int:
 
     L.lib()
     order = L.HASH_MORTON if args.hash == "morton" else L.HASH_LINEAR
-    sc = S.dambreak3d(dx=args.dx, hash_order=order)
+    hash_name = (f"Z-order over {1 << args.morton_xbits}x1x1 cell bricks" if args.hash == "morton"
+                 else "linear")
+    sc = S.dambreak3d(dx=args.dx, hash_order=order, morton_xbits=args.morton_xbits)
     n_total = sc.pos.shape[0]
     print(f"# workload: {sc.num_fluid} fluid + {sc.meta['num_wall']} wall particles, grid "
           f"{tuple(sc.params.grid_size)}, hash {args.hash}", file=sys.stderr, flush=True)
@@ -160,7 +165,7 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"3-D dam break, {sc.num_fluid:,} fluid + {sc.meta['num_wall']:,} wall particles, "
-                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {args.hash} hash, Tait, alpha=0.01"),
+                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {hash_name} cell hash, Tait, alpha=0.01"),
             "num_fluid": sc.num_fluid,
             "num_total": n_total,
             "grid": list(sc.params.grid_size),

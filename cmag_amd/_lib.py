@@ -68,7 +68,8 @@ class CmagParams(C.Structure):
         ("is_boundary_configuration", C.c_int32),
         ("wall_velocity", C.c_int32),
         ("morton_mask", C.c_uint32 * 3),
-        ("reserved", C.c_int32 * 5),
+        ("morton_xbits", C.c_uint32),
+        ("reserved", C.c_int32 * 4),
     ]
 
     def copy(self) -> "CmagParams":

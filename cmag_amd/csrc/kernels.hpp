@@ -19,6 +19,15 @@ int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, flo
                    const uint32_t* hash, const uint32_t* index, const float* oldPos,
                    const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s);
 
+// out = exclusive_scan(in) (in place allowed); ws holds scan_levels_words(len) words
+size_t scan_levels_words(size_t len);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws, hipStream_t s);
+
+// first[c] = number of particles in cells < c, first[C] = n (from the cell
+// ranges); ws holds scan_levels_words(C) words
+int launch_cell_offsets(const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t* first,
+                        uint32_t numCells, uint32_t n, uint32_t* ws, hipStream_t s);
+
 // stable LSD radix sort.  Sorts (keys, vals) using (alt_keys, alt_vals) as the
 // ping-pong buffers; *result_in_alt tells where the sorted pair ended up.
 size_t radix_sort_workspace_bytes(uint32_t n);
@@ -26,11 +35,19 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_
                      uint32_t n, int key_bits, void* workspace, size_t workspace_bytes,
                      bool* result_in_alt, hipStream_t s);
 
+// neighbour passes.  cellFirst (optional, from launch_cell_offsets) selects the
+// compact-offset walker; without it the reference cellStart/cellEnd are walked.
 int launch_density(const cmag_params& P, float* measures, const float* sortedPos,
-                   const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, hipStream_t s);
+                   const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, hipStream_t s,
+                   const uint32_t* cellFirst = nullptr);
 int launch_force(const cmag_params& P, float* acc, const float* measures, const float* sortedPos,
                  const float* sortedVel, const uint32_t* index, const uint32_t* cellStart,
-                 const uint32_t* cellEnd, uint32_t n, hipStream_t s);
+                 const uint32_t* cellEnd, uint32_t n, hipStream_t s,
+                 const uint32_t* cellFirst = nullptr);
+int launch_peristalsis_forces(const cmag_params& P, float* viscF, float* presF, const float* measures,
+                              const float* sortedPos, const float* sortedVel, const uint32_t* index,
+                              const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                              hipStream_t s, const uint32_t* cellFirst = nullptr);
 int launch_pressure_force(const cmag_params& P, float* presF, const float* measures,
                           const float* sortedPos, const uint32_t* index, const uint32_t* cellStart,
                           const uint32_t* cellEnd, uint32_t n, hipStream_t s);

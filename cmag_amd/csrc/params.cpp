@@ -66,15 +66,21 @@ static void finalize_impl(cmag_params* p) {
         while ((1u << b) < p->grid_size[a] && b < 31) ++b;
         p->grid_bits[a] = b;
     }
-    // Morton layout: bit b of every axis in x,y,z order, axes without a bit b skipped
+    // Morton layout: the low morton_xbits bits of x first (bricks of 2^xbits
+    // cells along x are contiguous), then bit b of every axis in x,y,z order
+    // with x counted from bit xbits, axes without a bit b skipped
     p->morton_mask[0] = p->morton_mask[1] = p->morton_mask[2] = 0;
-    uint32_t maxb = p->grid_bits[0];
-    if (p->grid_bits[1] > maxb) maxb = p->grid_bits[1];
-    if (p->grid_bits[2] > maxb) maxb = p->grid_bits[2];
+    uint32_t xb = p->morton_xbits < p->grid_bits[0] ? p->morton_xbits : p->grid_bits[0];
+    p->morton_xbits = xb;
+    uint32_t bits[3] = {p->grid_bits[0] - xb, p->grid_bits[1], p->grid_bits[2]};
     uint32_t bit = 0;
+    for (uint32_t b = 0; b < xb; ++b) p->morton_mask[0] |= 1u << bit++;
+    uint32_t maxb = bits[0];
+    if (bits[1] > maxb) maxb = bits[1];
+    if (bits[2] > maxb) maxb = bits[2];
     for (uint32_t b = 0; b < maxb; ++b)
         for (int a = 0; a < 3; ++a)
-            if (b < p->grid_bits[a]) p->morton_mask[a] |= 1u << bit++;
+            if (b < bits[a]) p->morton_mask[a] |= 1u << bit++;
     // kernel normalisations with the reference's float operation order
     const float h = p->smoothing_radius;
     if (p->dim == 3) {

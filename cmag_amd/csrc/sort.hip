@@ -84,9 +84,9 @@ __global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const uint32_t* __
 }
 
 // each thread owns SCAN_ITEMS consecutive elements; `carry` (may be null)
-// holds the exclusive base of every block
-__global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const uint32_t* __restrict__ in,
-                                                             uint32_t* __restrict__ out, size_t len,
+// holds the exclusive base of every block.  in == out is allowed: every thread
+// reads its own elements before the block barrier and writes them after it.
+__global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const uint32_t* in, uint32_t* out, size_t len,
                                                              const uint32_t* __restrict__ carry) {
     __shared__ uint32_t lds[SCAN_NT / 64];
     size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const uint32_t* __re
     }
 }
 
-static size_t scan_levels_words(size_t len) {
+size_t scan_levels_words(size_t len) {
     size_t words = 0;
     while (len > (size_t)SCAN_TILE) {
         len = (len + SCAN_TILE - 1) / SCAN_TILE;
@@ -119,7 +119,7 @@ static size_t scan_levels_words(size_t len) {
 }
 
 // out = exclusive_scan(in); workspace holds scan_levels_words(len) words
-static int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
                               hipStream_t s) {
     if (len == 0) return CMAG_OK;
     if (len <= (size_t)SCAN_TILE) {

@@ -14,13 +14,6 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-namespace cmag {
-int launch_peristalsis_forces(const cmag_params& P, float* viscF, float* presF, const float* measures,
-                              const float* sortedPos, const float* sortedVel, const uint32_t* index,
-                              const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
-                              hipStream_t s);
-}
-
 using namespace cmag;
 
 enum RefKind { REF_NONE = -1 };
@@ -37,6 +30,7 @@ struct cmag_system {
     float *measures = nullptr, *sortedPos = nullptr, *sortedVel = nullptr, *predicted = nullptr;
     uint32_t *hash = nullptr, *index = nullptr, *hashAlt = nullptr, *indexAlt = nullptr;
     uint32_t *cellStart = nullptr, *cellEnd = nullptr;
+    uint32_t *cellFirst = nullptr, *scanWs = nullptr;  // compact cell offsets for the walkers
     void* sortWs = nullptr;
     size_t sortWsBytes = 0;
 
@@ -78,7 +72,7 @@ struct cmag_system {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
-                        hash, index, hashAlt, indexAlt, cellStart, cellEnd, sortWs};
+                        hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -99,6 +93,8 @@ struct cmag_system {
         for (uint32_t** b : ub) CMAG_HIP_TRY(hipMalloc((void**)b, u));
         CMAG_HIP_TRY(hipMalloc((void**)&cellStart, sizeof(uint32_t) * (size_t)numCells));
         CMAG_HIP_TRY(hipMalloc((void**)&cellEnd, sizeof(uint32_t) * (size_t)numCells));
+        CMAG_HIP_TRY(hipMalloc((void**)&cellFirst, sizeof(uint32_t) * ((size_t)numCells + 1)));
+        CMAG_HIP_TRY(hipMalloc((void**)&scanWs, sizeof(uint32_t) * (scan_levels_words(numCells) + 1)));
         sortWsBytes = radix_sort_workspace_bytes(n);
         CMAG_HIP_TRY(hipMalloc(&sortWs, sortWsBytes));
         float* zb[] = {pos, vel, velLF, acc, measures, sortedPos, sortedVel, presF, predicted};
@@ -136,15 +132,16 @@ struct cmag_system {
         CMAG_TRY(mark(CMAG_STAGE_REORDER));
         CMAG_TRY(launch_reorder(cellStart, cellEnd, sortedPos, sortedVel, hash, index, pos, velLF, n,
                                 numCells, stream));
+        CMAG_TRY(launch_cell_offsets(cellStart, cellEnd, cellFirst, numCells, n, scanWs, stream));
         CMAG_TRY(mark(CMAG_STAGE_DENSITY));
-        CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream));
+        CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
         CMAG_TRY(mark(CMAG_STAGE_FORCE));
         if (P.system == CMAG_SYS_PERISTALSIS) {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
-                                               cellStart, cellEnd, n, stream));
+                                               cellStart, cellEnd, n, stream, cellFirst));
         } else {
             CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
-                                  stream));
+                                  stream, cellFirst));
         }
         CMAG_TRY(mark(CMAG_STAGE_INTEGRATE));
         CMAG_TRY(launch_integrate(P, pos, vel, velLF, acc, presF, wave_time, n, stream));

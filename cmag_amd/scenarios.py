@@ -259,14 +259,16 @@ def poiseuille_series(y: np.ndarray, t: float, F: float, l: float, nu: float,
 # ---------------------------------------------------------------- dam break
 def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), layers: int = 3,
                h_factor: float = 1.3, alpha: float = 0.01, jitter: float = 0.01, seed: int = 1973,
-               grid=None, hash_order: int = L.HASH_LINEAR, cell_factor: float = 1.0) -> Scenario:
+               grid=None, hash_order: int = L.HASH_LINEAR, cell_factor: float = 1.0,
+               morton_xbits: int = 0) -> Scenario:
     """3-D dam break in the DamBreak system's loop bodies (Tait EOS, Monaghan
     artificial viscosity, Lennard-Jones wall particles, clamp walls).
 
     Water column column[0] x column[1] x column[2] against the x=0 wall of a
     tank tank[0] x tank[1] x tank[2]; `layers` wall layers on the floor and the
     four sides (open top); lattice spacing dx with seeded jitter +-jitter*dx.
-    Cells are cell_factor * 2h wide (cellcount = ceil(2h / cell))."""
+    Cells are cell_factor * 2h wide (cellcount = ceil(2h / cell)); with the
+    Morton hash, morton_xbits > 0 orders bricks of 2^xbits cells along x."""
     h = h_factor * dx
     cs = cell_factor * 2.0 * h
     cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
@@ -276,6 +278,7 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
         grid = tuple(_pow2_at_least(e / cs + 1) for e in ext)
     p = _blank(L.SYS_DAMBREAK, 3)
     p.hash_order = hash_order
+    p.morton_xbits = morton_xbits
     p.eos = L.EOS_TAIT
     p.cellcount = cellcount
     for a in range(3):

@@ -43,7 +43,7 @@ enum cmag_eos {
 /* cell index -> hash key */
 enum cmag_hash_order {
     CMAG_HASH_LINEAR = 0, /* (z*gy)*gx + y*gx + x, fluid_kernel.cu:27-32 */
-    CMAG_HASH_MORTON = 1  /* bit-interleaved x,y,z (x lowest) */
+    CMAG_HASH_MORTON = 1  /* Z order: low morton_xbits bits of x, then x,y,z interleaved */
 };
 
 /* position -> cell coordinate rounding rule (bit-exactness depends on it) */
@@ -114,7 +114,8 @@ typedef struct cmag_params {
     int32_t  wall_velocity;    /* 1: moving walls carry their kinematic velocity */
 
     uint32_t morton_mask[3];   /* key bits owned by each axis (Morton order) */
-    int32_t  reserved[5];
+    uint32_t morton_xbits;     /* Morton order over 2^xbits x 1 x 1 cell bricks (0 = plain) */
+    int32_t  reserved[4];
 } cmag_params;
 
 #ifdef __cplusplus

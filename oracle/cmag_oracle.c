@@ -73,20 +73,25 @@ void oracle_grid_pos(const cmag_params* P, float x, float y, float z, int32_t ou
 }
 
 /* calcGridHash, fluid_kernel.cu:27-32: mask each axis with gridSize-1 (the
- * two's-complement wrap), then linear order; Morton order interleaves the
- * masked coordinates bit by bit, x lowest, skipping axes that ran out of bits. */
+ * two's-complement wrap), then linear order.  Z order: the low xbits bits of
+ * x, followed by the interleave of (x >> xbits, y, z) bit by bit, x first,
+ * skipping axes that ran out of bits -- a Morton curve over bricks of
+ * 2^xbits x 1 x 1 cells (xbits = 0 is the plain curve). */
 uint32_t oracle_grid_hash(const cmag_params* P, const int32_t g[3]) {
     uint32_t c[3];
     for (int a = 0; a < 3; ++a) c[a] = (uint32_t)g[a] & (P->grid_size[a] - 1u);
     if (P->hash_order == CMAG_HASH_LINEAR)
         return c[2] * P->grid_size[1] * P->grid_size[0] + c[1] * P->grid_size[0] + c[0];
-    uint32_t key = 0, bit = 0;
-    uint32_t maxb = P->grid_bits[0];
-    if (P->grid_bits[1] > maxb) maxb = P->grid_bits[1];
-    if (P->grid_bits[2] > maxb) maxb = P->grid_bits[2];
+    uint32_t xb = P->morton_xbits < P->grid_bits[0] ? P->morton_xbits : P->grid_bits[0];
+    uint32_t coord[3] = {c[0] >> xb, c[1], c[2]};
+    uint32_t bits[3] = {P->grid_bits[0] - xb, P->grid_bits[1], P->grid_bits[2]};
+    uint32_t key = c[0] & ((1u << xb) - 1u), bit = xb;
+    uint32_t maxb = bits[0];
+    if (bits[1] > maxb) maxb = bits[1];
+    if (bits[2] > maxb) maxb = bits[2];
     for (uint32_t b = 0; b < maxb; ++b)
         for (int a = 0; a < 3; ++a)
-            if (b < P->grid_bits[a]) key |= ((c[a] >> b) & 1u) << bit++;
+            if (b < bits[a]) key |= ((coord[a] >> b) & 1u) << bit++;
     return key;
 }
 

@@ -102,6 +102,24 @@ def test_morton_unequal_bits_is_bijection(oracle):
     assert oracle.grid_hash(p, (-1, 0, 0)) == oracle.grid_hash(p, (15, 0, 0))
 
 
+def test_brick_morton_order(oracle):
+    """Z order over 4x1x1 bricks (xbits = 2): the four cells of a brick are
+    consecutive keys, and brick coordinates follow the plain Z order."""
+    p = grid_params(grid=(16, 8, 4), order=L.HASH_MORTON)
+    p.morton_xbits = 2
+    S.finalize(p)
+    q = grid_params(grid=(4, 8, 4), order=L.HASH_MORTON)   # the brick grid
+    keys = {}
+    for x in range(16):
+        for y in range(8):
+            for z in range(4):
+                k = oracle.grid_hash(p, (x, y, z))
+                keys[(x, y, z)] = k
+                assert k == (x & 3) | (oracle.grid_hash(q, (x >> 2, y, z)) << 2)
+    assert sorted(keys.values()) == list(range(16 * 8 * 4))
+    assert p.morton_mask[0] & 3 == 3
+
+
 def test_calc_hash_index_identity(oracle):
     p = grid_params()
     rng = np.random.default_rng(2)
