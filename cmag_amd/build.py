@@ -82,6 +82,24 @@ def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
+EXAMPLES = ["poiseuille_report"]
+
+
+def build_examples() -> list[Path]:
+    """C++ programs written against include/cmag/systems.hpp, linked to the library."""
+    out = []
+    for name in EXAMPLES:
+        src = PKG / "examples" / f"{name}.cpp"
+        exe = PKG / "examples" / name
+        cmd = [hipcc(), "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(src), "-o", str(exe),
+               f"-L{PKG}", "-lcmag_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/.."]
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            raise RuntimeError(f"example {name} failed to build:\n{proc.stderr}")
+        out.append(exe)
+    return out
+
+
 def build_oracle() -> Path:
     """The oracle is test infrastructure; built beside the library for the tests."""
     odir = ROOT / "oracle"
@@ -98,6 +116,7 @@ def main() -> None:
     args = ap.parse_args()
     print(build_library(verbose=args.v))
     if not args.lib:
+        print(build_examples())
         print(build_oracle())
 
 

@@ -1,272 +1,17 @@
-// sph.hip - neighbour passes of the WCSPH step (gfx950, wave64, FP32).
-//
-// One thread per particle in sorted order; the stencil of (2*cellcount+1)^dim
-// cells is walked in the reference's z, y, x order and each cell's particles in
-// sorted order, so the sums accumulate in the reference's order.  Candidates
-// are culled on r^2 < 4h^2 before any square root; inner-loop reciprocals use
-// the hardware v_rcp/v_sqrt (1 ulp), the per-particle divisions stay IEEE.
-//
-//   density       fluid_kernel.cu:99-169, poiseuilleFlowKernel.cu:99-182,
-//                 PeristalsisDensity.cu:3-79
-//   force         fluid_kernel.cu:172-292 (DamBreak: pressure + Monaghan AV,
-//                 Lennard-Jones walls), poiseuilleFlowKernel.cu:184-309
-//                 (pressure + Morris viscosity with wall extrapolation)
-//   pressure /    PeristalsisPressureForce.cu:3-90,
-//   viscous       PeristalsisViscousForce.cu:143-241
-#include "common.hpp"
+// sph.hip - neighbour-pass kernels of one system and their launchers.
+// The per-particle bodies live in sph_device.hpp (shared with batch.hip).
 #include "kernels.hpp"
+#include "sph_device.hpp"
 
 namespace cmag {
 
-__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
-// L2).  Renumbering them so XCD k runs one contiguous block range keeps the
-// neighbour particles a workgroup reads in the L2 of the XCD that reads them.
-__device__ __forceinline__ uint32_t xcd_block() {
-    constexpr uint32_t XCDS = 8;
-    const uint32_t nb = gridDim.x, b = blockIdx.x;
-    const uint32_t k = b % XCDS, slot = b / XCDS;
-    const uint32_t base = nb / XCDS, extra = nb % XCDS;
-    return k * base + min(k, extra) + slot;
-}
-
-// integer power by squaring (gamma is an integer in [1,15])
-__device__ __forceinline__ float pow_int(float x, int e) {
-    float r = 1.0f, b = x;
-    while (e) {
-        if (e & 1) r *= b;
-        b *= b;
-        e >>= 1;
-    }
-    return r;
-}
-
-__device__ __forceinline__ float eos(const cmag_params& P, float dens) {
-    if (P.eos == CMAG_EOS_TAIT) return P.B * (pow_int(dens / P.rest_density, (int)P.gamma) - 1.0f);
-    if (P.eos == CMAG_EOS_LINEAR_C2) return P.soundspeed * P.soundspeed * dens;
-    return 50.0f * P.soundspeed * dens;
-}
-
-// Range policies of the stencil walker.  A run holds the cells with keys
-// k0..k1 (consecutive keys are contiguous in the sorted arrays).
-//   RefCells   the reference arrays: first non-empty cellStart and last
-//              cellEnd of the run (empty cells hold 0xffffffff)
-//   FirstCells compact offsets first[c] (cells before c), one pair of loads
-struct RefCells {
-    const uint32_t* __restrict__ cellStart;
-    const uint32_t* __restrict__ cellEnd;
-    uint32_t n;
-    __device__ __forceinline__ bool range(uint32_t k0, uint32_t k1, uint32_t& s, uint32_t& e) const {
-        s = CMAG_CELL_EMPTY;
-        e = 0;
-        for (uint32_t k = k0; k <= k1; ++k) {
-            const uint32_t st = cellStart[k];
-            if (st == CMAG_CELL_EMPTY) continue;
-            if (s == CMAG_CELL_EMPTY) s = st;
-            // clamped to the particle count: consistent cell arrays never
-            // need it, inconsistent ones cannot read past n
-            e = min(cellEnd[k], n);
-        }
-        return s != CMAG_CELL_EMPTY && s < e;
-    }
-};
-
-struct FirstCells {
-    const uint32_t* __restrict__ first;
-    __device__ __forceinline__ bool range(uint32_t k0, uint32_t k1, uint32_t& s, uint32_t& e) const {
-        s = first[k0];
-        e = first[k1 + 1];
-        return s < e;
-    }
-};
-
-// A run of key-consecutive cells: a contiguous range of the sorted arrays,
-// with the target position minus the run's periodic image offset and the
-// image-shift code (sx+1) + 3(sy+1) + 9(sz+1) (13: no shift).
-struct Run {
-    uint32_t s, len;
-    float px;
-    int code;
-};
-
-// Walk the stencil of gp in z, y, x order and call cand(j, qx, qy, qz, code)
-// for every candidate j in the reference's order.  Along x, cells with
-// consecutive keys form one run (linear order: the whole row; brick Z order:
-// the part inside one brick; plain Z order: an even/odd pair); a row has at
-// most two runs except at periodic seams.  The runs of a row are gathered
-// first and then walked as one loop, so every lane of a wave spends the row in
-// the same loop whatever its x position -- lanes whose row splits at a
-// different x would otherwise serialise the wave.
-template <int DIM, int HASH, typename Cells, typename C>
-__device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4& pi, const Cell& gp,
-                                             const Cells& cells, C&& cand) {
-    const int cc = P.cellcount;
-    const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
-    const uint32_t gxm = P.grid_size[0] - 1u, gym = P.grid_size[1] - 1u, gzm = P.grid_size[2] - 1u;
-    const uint32_t Mx = P.morton_mask[0], My = P.morton_mask[1], Mz = P.morton_mask[2];
-    const bool perx = P.periodic_mask & 1, pery = P.periodic_mask & 2, perz = P.periodic_mask & 4;
-    uint32_t kx0 = 0, ky0 = 0, kz0 = 0;
-    if (HASH == CMAG_HASH_MORTON) {
-        const uint32_t own = grid_hash(P, gp.x, gp.y, gp.z);
-        kx0 = own & Mx;
-        ky0 = own & My;
-        kz0 = own & Mz;
-        for (int k = 0; k < cc; ++k) {
-            kx0 = dilated_dec(kx0, Mx);
-            ky0 = dilated_dec(ky0, My);
-            if (DIM == 3) kz0 = dilated_dec(kz0, Mz);
-        }
-    }
-    auto run_pair = [&](const Run& a, const Run& b, float py, float pz) {
-        const uint32_t total = a.len + b.len;
-        for (uint32_t t = 0; t < total; ++t) {
-            const bool second = t >= a.len;
-            const uint32_t j = second ? b.s + (t - a.len) : a.s + t;
-            cand(j, second ? b.px : a.px, py, pz, second ? b.code : a.code);
-        }
-    };
-    uint32_t kz = kz0;
-    for (int dz = zlo; dz <= zhi; ++dz) {
-        const int nz = gp.z + dz;
-        const int sz = perz ? image_shift(nz, P.grid_size[2]) : 0;
-        const float pz = perz ? pi.z - (float)sz * P.world_size[2] : pi.z;
-        uint32_t ky = ky0;
-        for (int dy = -cc; dy <= cc; ++dy) {
-            const int ny = gp.y + dy;
-            const int sy = pery ? image_shift(ny, P.grid_size[1]) : 0;
-            const float py = pery ? pi.y - (float)sy * P.world_size[1] : pi.y;
-            const int code_yz = 3 * (sy + 1) + 9 * (sz + 1);
-            const uint32_t row = (HASH == CMAG_HASH_LINEAR)
-                                     ? (((uint32_t)nz & gzm) * P.grid_size[1] + ((uint32_t)ny & gym)) * P.grid_size[0]
-                                     : (kz | ky);
-            Run a{0u, 0u, pi.x, 13}, b{0u, 0u, pi.x, 13};
-            bool have_a = false;
-            auto emit = [&](uint32_t k0, uint32_t k1, float px, int code) {
-                uint32_t s, e;
-                if (!cells.range(k0, k1, s, e)) return;
-                const Run r{s, e - s, px, code};
-                if (!have_a) {
-                    a = r;
-                    have_a = true;
-                } else if (b.len == 0) {
-                    b = r;
-                } else {  // a third run in one row (periodic seam with plain Z order)
-                    run_pair(a, b, py, pz);
-                    a = r;
-                    b.len = 0;
-                }
-            };
-            uint32_t k0 = 0, k1 = 0;
-            float run_px = pi.x;
-            int run_sx = 0;
-            uint32_t kx = kx0;
-            for (int dx = -cc; dx <= cc; ++dx) {
-                const int nx = gp.x + dx;
-                const int sx = perx ? image_shift(nx, P.grid_size[0]) : 0;
-                const float px = perx ? pi.x - (float)sx * P.world_size[0] : pi.x;
-                const uint32_t key = (HASH == CMAG_HASH_LINEAR) ? row + ((uint32_t)nx & gxm) : (row | kx);
-                if (dx == -cc) {
-                    k0 = k1 = key;
-                    run_px = px;
-                    run_sx = sx;
-                } else if (key == k1 + 1u && sx == run_sx) {
-                    k1 = key;
-                } else {
-                    emit(k0, k1, run_px, (run_sx + 1) + code_yz);
-                    k0 = k1 = key;
-                    run_px = px;
-                    run_sx = sx;
-                }
-                if (HASH == CMAG_HASH_MORTON) kx = dilated_inc(kx, Mx);
-            }
-            emit(k0, k1, run_px, (run_sx + 1) + code_yz);
-            run_pair(a, b, py, pz);
-            if (HASH == CMAG_HASH_MORTON) ky = dilated_inc(ky, My);
-        }
-        if (HASH == CMAG_HASH_MORTON) kz = dilated_inc(kz, Mz);
-    }
-}
-
-// ---------------------------------------------------------------------------
 template <int SYS, int DIM, int HASH, typename Cells>
 __global__ __launch_bounds__(256) void density_kernel(const cmag_params P, float4* __restrict__ measures,
                                                        const float4* __restrict__ sortedPos, const Cells cells,
                                                        uint32_t n) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 pi = sortedPos[i];
-    float2* m2 = reinterpret_cast<float2*>(measures);
-    if (SYS == CMAG_SYS_POISEUILLE && pi.w == CMAG_PO_WALL) {
-        m2[2 * (size_t)i] = make_float2(P.rest_density, eos(P, P.rest_density));
-        return;
-    }
-    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
-    float sum = 0.0f;
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int) {
-        const float4 pj = sortedPos[j];
-        const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < r2max) {
-            const float q = fsqrt(r2) * inv_h;
-            const float t = 1.0f - 0.5f * q;
-            const float t2 = t * t;
-            sum += t2 * t2 * (2.0f * q + 1.0f);
-        }
-    });
-    const float dens = sum * P.kernel_coeff * P.particle_mass;
-    if (SYS == CMAG_SYS_PERISTALSIS)
-        measures[i] = make_float4(dens, eos(P, dens), 0.0f, pi.w);
-    else
-        m2[2 * (size_t)i] = make_float2(dens, eos(P, dens));
-}
-
-// getVelocityDiff, poiseuilleFlowKernel.cu:184-208
-__device__ __forceinline__ float3 vel_diff_poiseuille(const cmag_params& P, const float4& vi,
-                                                      const float4& pi, const float4& vj,
-                                                      const float4& pj) {
-    if (pj.w == CMAG_PO_WALL && pj.y > P.wall_top) {
-        float beta = fminf(P.beta_max, 1.0f + (pj.y - P.wall_top) / (P.wall_top - pi.y));
-        return make_float3(beta * vi.x, beta * vi.y, beta * vi.z);
-    }
-    if (pj.w == CMAG_PO_WALL && pj.y < P.wall_bottom) {
-        float beta = fminf(P.beta_max, 1.0f + (P.wall_bottom - pj.y) / (pi.y - P.wall_bottom));
-        return make_float3(beta * vi.x, beta * vi.y, beta * vi.z);
-    }
-    return make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
-}
-
-// ---- force passes: two phases per target ------------------------------------
-// Phase 1 walks the stencil and only tests candidates (r^2 < 4h^2), appending
-// the in-range ones to a per-lane list in LDS; phase 2 evaluates the pair
-// terms of the listed neighbours.  Without the split, the ~50-instruction pair
-// term would run for every candidate (some lane of the wave is almost always
-// in range), i.e. ~475 times per particle instead of ~74.  The list holds
-// NB_CAP entries per lane; when a lane fills it, every active lane flushes.
-constexpr int NB_CAP = 16;
-constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
-
-struct NbEntry {
-    __device__ static __forceinline__ uint32_t pack(uint32_t j, int code) {
-        return j | ((uint32_t)code << NB_SHIFT_BITS);
-    }
-    __device__ static __forceinline__ uint32_t index(uint32_t e) { return e & ((1u << NB_SHIFT_BITS) - 1u); }
-    __device__ static __forceinline__ int code(uint32_t e) { return (int)(e >> NB_SHIFT_BITS); }
-};
-
-// target position minus the periodic image offset named by code, computed with
-// the same float operations as walk_stencil
-__device__ __forceinline__ float3 image_target(const cmag_params& P, const float4& pi, int code) {
-    const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
-    float3 q;
-    q.x = (P.periodic_mask & 1) ? pi.x - (float)sx * P.world_size[0] : pi.x;
-    q.y = (P.periodic_mask & 2) ? pi.y - (float)sy * P.world_size[1] : pi.y;
-    q.z = (P.periodic_mask & 4) ? pi.z - (float)sz * P.world_size[2] : pi.z;
-    return q;
+    density_body<SYS, DIM, HASH>(P, i, measures, sortedPos, cells);
 }
 
 template <int SYS, int DIM, int HASH, typename Cells>
@@ -279,84 +24,7 @@ __global__ __launch_bounds__(256) void force_kernel(const cmag_params P, float4*
     __shared__ uint32_t nbl[256 / 64][NB_CAP][64];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const float4 pi = sortedPos[i];
-    if (SYS == CMAG_SYS_DAMBREAK && pi.w != CMAG_DB_FLUID) return;
-    const float4 vi = sortedVel[i];
-    const float4 mi = measures[i];
-    const float dens = mi.x;
-    const float pterm_i = mi.y / (dens * dens);
-    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
-    const float m = P.particle_mass;
-    const float gc = P.grad_coeff;
-    const float av_k = 2.0f * P.alpha * h * P.soundspeed;  // nu * (rho_i + rho_j)
-    const float av_eps = 0.001f * h * h;
-    const float visc_k = m * (P.mu + P.mu);
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    int count = 0;
-
-    auto flush = [&]() {
-        for (int k = 0; k < NB_CAP; ++k) {
-            if (k >= count) break;
-            const uint32_t e = nbl[w][k][lane];
-            const uint32_t j = NbEntry::index(e);
-            const float3 q = image_target(P, pi, NbEntry::code(e));
-            const float4 pj = sortedPos[j];
-            const float4 mj = measures[j];
-            const float4 vj = sortedVel[j];
-            const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float dist = fsqrt(r2);
-            const float inv_d = frcp(dist);
-            const float qq = dist * inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float t2 = t * t;
-            const float temp = gc * (t2 * t2 - t2 * t * (2.0f * qq + 1.0f));
-            const float inv_dj = frcp(mj.x);
-            float s = pterm_i + mj.y * inv_dj * inv_dj;
-            if (SYS == CMAG_SYS_DAMBREAK) {
-                const float vr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-                if (vr < 0.0f) s += -av_k * frcp(dens + mj.x) * vr * frcp(r2 + av_eps);
-                const float kk = -m * s * inv_d * temp;
-                fx += kk * rx; fy += kk * ry; fz += kk * rz;
-            } else {
-                const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
-                const float kk = -m * s * inv_d * temp;
-                const float kv = visc_k * frcp(dens * mj.x) * inv_d * temp;
-                fx += kk * rx + kv * V.x;
-                fy += kk * ry + kv * V.y;
-                fz += kk * rz + kv * V.z;
-            }
-        }
-        count = 0;
-    };
-
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int code) {
-        const float4 pj = sortedPos[j];
-        const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (j != i) {
-            if (SYS == CMAG_SYS_DAMBREAK && pj.w != CMAG_DB_FLUID) {
-                // Lennard-Jones wall term: every wall particle of the stencil acts
-                const float dist = fsqrt(r2);
-                const float ad = P.lj_a * frcp(dist);
-                const float ad2 = ad * ad;
-                const float ad6 = ad2 * ad2 * ad2;
-                const float sc = P.lj_D * (ad6 * ad6 - ad6) * frcp(dist * dist);
-                fx += sc * rx; fy += sc * ry; fz += sc * rz;
-            } else if (r2 < r2max) {
-                nbl[w][count][lane] = NbEntry::pack(j, code);
-                ++count;
-            }
-        }
-        if (__ballot(count == NB_CAP)) flush();
-    });
-    flush();
-    const uint32_t k = index[i];
-    if (k < n) acc[k] = make_float4(fx, fy, fz, 0.0f);
+    force_body<SYS, DIM, HASH>(P, i, acc, measures, sortedPos, sortedVel, index, cells, n, nbl[threadIdx.x >> 6]);
 }
 
 // Peristalsis: pressure and viscous passes; the fused variant produces both
@@ -370,68 +38,8 @@ __global__ __launch_bounds__(256) void peristalsis_force_kernel(
     __shared__ uint32_t nbl[256 / 64][NB_CAP][64];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const float4 pi = sortedPos[i];
-    const float4 mi = measures[i];
-    const float dens = mi.x;
-    const float pterm_i = mi.y / (dens * dens);
-    float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (WANT_VISC) vi = sortedVel[i];
-    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
-    const float m = P.particle_mass;
-    const float gc = P.grad_coeff;
-    const float visc_k = m * (P.mu + P.mu);
-    float vx = 0.f, vy = 0.f, vz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
-    int count = 0;
-
-    auto flush = [&]() {
-        for (int k = 0; k < NB_CAP; ++k) {
-            if (k >= count) break;
-            const uint32_t e = nbl[w][k][lane];
-            const uint32_t j = NbEntry::index(e);
-            const float3 q = image_target(P, pi, NbEntry::code(e));
-            const float4 pj = sortedPos[j];
-            const float4 mj = measures[j];
-            const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float dist = fsqrt(r2);
-            const float inv_d = frcp(dist);
-            const float qq = dist * inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float t2 = t * t;
-            const float temp = gc * (t2 * t2 - t2 * t * (2.0f * qq + 1.0f));
-            if (WANT_PRES) {
-                const float inv_dj = frcp(mj.x);
-                const float kk = -m * temp * (pterm_i + mj.y * inv_dj * inv_dj) * inv_d;
-                px += kk * rx; py += kk * ry; pz += kk * rz;
-            }
-            if (WANT_VISC) {
-                const float4 vj = sortedVel[j];
-                const float kv = visc_k * temp * frcp(dens * mj.x * dist);
-                vx += kv * (vi.x - vj.x); vy += kv * (vi.y - vj.y); vz += kv * (vi.z - vj.z);
-            }
-        }
-        count = 0;
-    };
-
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float qx, float qy, float qz, int code) {
-        const float4 pj = sortedPos[j];
-        const float rx = qx - pj.x, ry = qy - pj.y, rz = qz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (j != i && r2 < r2max) {
-            nbl[w][count][lane] = NbEntry::pack(j, code);
-            ++count;
-        }
-        if (__ballot(count == NB_CAP)) flush();
-    });
-    flush();
-    const uint32_t k = index[i];
-    if (k >= n) return;
-    if (WANT_VISC) viscF[k] = make_float4(vx, vy, vz, 0.0f);
-    if (WANT_PRES) presF[k] = make_float4(px, py, pz, 0.0f);
+    peristalsis_force_body<DIM, HASH, WANT_VISC, WANT_PRES>(P, i, viscF, presF, measures, sortedPos, sortedVel,
+                                                            index, cells, n, nbl[threadIdx.x >> 6]);
 }
 
 // ---- launchers --------------------------------------------------------------

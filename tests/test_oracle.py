@@ -682,3 +682,29 @@ def test_poiseuille_channel_matches_transient_series(oracle):
     k = lambda h, dx: S.lattice_laplacian_moment(h, dx, 2)  # noqa: E731
     assert _channel_profile_error(oracle, 1.3, k) < 0.01
     assert _channel_profile_error(oracle, 1.3, lambda h, dx: 1.0) < 0.02
+
+
+def report_profile(pos, vel, p, r):
+    """Poiseuille.Report/dump.h:68-81: fluid particles of the column 0 < x < 2r,
+    height above the bottom wall"""
+    fl = (pos[:, 3] == 0) & (pos[:, 0] > 0) & (pos[:, 0] < 2 * r)
+    y = pos[fl, 1] + abs(float(p.world_origin[1])) - 3 * 2 * r
+    return y, vel[fl, 0]
+
+
+def test_reference_poiseuille_report_matches_series(oracle):
+    """The reference's own validation run (Poiseuille.Report/dump.h: 16x58 fluid,
+    3 wall layers, r = 1/116000, parity constants) against the analytic series
+    it is plotted with (XVelocityYPosition.m:10-12, F = 1e-4, l = 1e-3,
+    nu = 1e-6) at the report's times: within 1% L2, FP32 oracle."""
+    r = 1.0 / (2 * 58 * 1000)
+    p = S.params_poiseuille((16, 58, 1), 3, (16, 64, 4), r)
+    n = 16 * 58 + 2 * 16 * 3
+    sim = oracle.Simulation(p, oracle.poiseuille_init(p, n))
+    for t in (0.0225, 0.045, 0.1125, 0.225):
+        while sim.elapsed < t:
+            sim.update(1)
+        y, u = report_profile(sim.pos, sim.vel, p, r)
+        assert y.size == 58
+        want = S.poiseuille_series(y, sim.elapsed, 1e-4, 1e-3, 1e-6)
+        assert np.linalg.norm(u - want) / np.linalg.norm(want) < 0.01
