@@ -55,6 +55,47 @@ def cell_bytes(stage: str, num_cells: int, occupied: int) -> int:
     return 0
 
 
+def count_pairs(sysobj, L) -> int:
+    """in-range neighbour pairs (j != i, |r| < 2h) of the current sorted state"""
+    import ctypes as C
+
+    import torch
+
+    n = sysobj.getNumParticles()
+    counts = torch.empty(n, dtype=torch.int32, device="cuda")
+    p = sysobj.params
+    sysobj.synchronize()
+    L.call("cmag_count_neighbours", C.byref(p), counts.data_ptr(), sysobj.device_ptr(L.ARRAY_SORTEDPOS),
+           sysobj.device_ptr(L.ARRAY_CELLSTART), sysobj.device_ptr(L.ARRAY_CELLEND), n, None)
+    torch.cuda.synchronize()
+    return int(counts.sum(dtype=torch.int64).item())
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 5) -> None:
+    """plain device copies of known size: the calibration for rocprof's
+    FETCH_SIZE / WRITE_SIZE counters (one copy moves nbytes in and out)"""
+    import torch
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+
+
+def measured_traffic(workload_key: str, kernel: str) -> float | None:
+    """HBM bytes per launch of kernel from the committed rocprof counter
+    summary of this workload, or None"""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        rec = json.loads(f.read_text()).get(workload_key, {})
+        return rec.get("kernels", {}).get(kernel, {}).get("hbm_bytes_per_launch")
+    except (ValueError, AttributeError):
+        return None
+
+
 def cpu_baseline(target_seconds: float = 12.0) -> dict:
     from cmag_amd import scenarios as S
     from oracle import oracle as O
@@ -96,6 +137,8 @@ def main() -> int:
     ap.add_argument("--morton-xbits", type=int, default=4,
                     help="Z order over bricks of 2^xbits cells along x (0 = plain Morton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, run 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
@@ -124,6 +167,7 @@ def main() -> int:
     occupied = int(np.count_nonzero(cs != L.CELL_EMPTY))
     num_cells = cs.size
     del cs
+    pairs = count_pairs(sysobj, L)
 
     sysobj.enable_timing(True)
     sysobj.stage_times()
@@ -135,6 +179,8 @@ def main() -> int:
     t1 = time.perf_counter()
     stages = sysobj.stage_times()
     sysobj.enable_timing(False)
+    if args.copy_probe:
+        copy_probe()
     wall_s = t1 - t0
     ms_per_step = 1000.0 * wall_s / args.steps
 
@@ -149,6 +195,8 @@ def main() -> int:
     achieved = d["alg_bytes"] / (d["avg_ms"] * 1e-3) / 1e9
     step_bytes = sum(v["alg_bytes"] for v in per_stage.values())
     print(f"# stages: {json.dumps(per_stage)}", file=sys.stderr, flush=True)
+    workload_key = f"dambreak3d_dx{args.dx:.6g}_{args.hash}{args.morton_xbits if args.hash == 'morton' else ''}"
+    traffic = measured_traffic(workload_key, dominant)
 
     result = {
         "metric": "particle-steps/s + HBM GB/s at 16M particles; batched & adjoint vs plain",
@@ -173,6 +221,9 @@ def main() -> int:
             "step_alg_bytes": step_bytes,
             "step_alg_GBs": round(step_bytes * args.steps / wall_s / 1e9, 1),
             "stages_ms": {k: v["avg_ms"] for k, v in per_stage.items()},
+            "neighbour_pairs_per_pass": pairs,
+            # in-range ordered pairs (j != i) of the density and force passes per second
+            "neighbour_pairs_per_s": 2.0 * pairs * args.steps / wall_s,
         },
         "roofline": {
             "bound": "hbm",
@@ -181,7 +232,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
     }
     if not args.no_cpu_baseline:

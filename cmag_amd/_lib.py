@@ -131,6 +131,7 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_sort_by_key": (C.c_int, [_vp, _vp, _u32, C.c_int, _vp, C.c_size_t, _vp]),
     "cmag_reorder_find_cell_start": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "cmag_density": (C.c_int, [_P, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "cmag_count_neighbours": (C.c_int, [_P, _vp, _vp, _vp, _vp, _u32, _vp]),
     "cmag_force": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "cmag_pressure_force": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
     "cmag_viscous_force": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),

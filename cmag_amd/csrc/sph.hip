@@ -42,6 +42,26 @@ __global__ __launch_bounds__(256) void peristalsis_force_kernel(
                                                             index, cells, n, nbl[threadIdx.x >> 6]);
 }
 
+// in-range neighbours (j != i, r^2 < 4h^2) of each sorted particle: the pair
+// count behind the neighbour-pair interactions/s figure
+template <int SYS, int DIM, int HASH, typename Cells>
+__global__ __launch_bounds__(256) void count_kernel(const cmag_params P, uint32_t* __restrict__ counts,
+                                                     const float4* __restrict__ sortedPos, const Cells cells,
+                                                     uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = sortedPos[i];
+    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
+    const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
+    uint32_t c = 0;
+    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int) {
+        const float4 pj = sortedPos[j];
+        const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
+        c += (j != i && rx * rx + ry * ry + rz * rz < r2max) ? 1u : 0u;
+    });
+    counts[i] = c;
+}
+
 // ---- launchers --------------------------------------------------------------
 // every kernel is instantiated per (system, dimension, hash order, range policy)
 template <template <int, int, int> class K, typename... Args>
@@ -83,6 +103,15 @@ int launch_density(const cmag_params& P, float* measures, const float* sortedPos
     dispatch3<DensityGo>(P, P, measures, sortedPos, cellStart, cellEnd, cellFirst, n, s);
     return check_launch("density_kernel");
 }
+
+template <int SYS, int DIM, int H>
+struct CountGo {
+    static void go(const cmag_params& P, uint32_t* counts, const float* sortedPos, const uint32_t* cellStart,
+                   const uint32_t* cellEnd, uint32_t n, hipStream_t s) {
+        hipLaunchKernelGGL((count_kernel<SYS, DIM, H, RefCells>), dim3(div_up(n, 256)), dim3(256), 0, s, P,
+                           counts, reinterpret_cast<const float4*>(sortedPos), RefCells{cellStart, cellEnd, n}, n);
+    }
+};
 
 template <int SYS, int DIM, int H>
 struct ForceGo {
@@ -190,6 +219,17 @@ int cmag_density(const cmag_params* p, float* measures, const float* sortedPos,
     if (n && (!measures || !sortedPos || !cellStart || !cellEnd))
         return fail(CMAG_E_INVALID, "cmag_density: null pointer");
     return launch_density(*p, measures, sortedPos, cellStart, cellEnd, n, as_stream(stream));
+}
+
+int cmag_count_neighbours(const cmag_params* p, uint32_t* counts, const float* sortedPos,
+                          const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                          cmag_stream_t stream) {
+    CMAG_TRY(validate_params(p));
+    if (n && (!counts || !sortedPos || !cellStart || !cellEnd))
+        return fail(CMAG_E_INVALID, "cmag_count_neighbours: null pointer");
+    if (n == 0) return CMAG_OK;
+    dispatch3<CountGo>(*p, *p, counts, sortedPos, cellStart, cellEnd, n, as_stream(stream));
+    return check_launch("count_kernel");
 }
 
 int cmag_force(const cmag_params* p, float* acc, const float* measures, const float* sortedPos,

@@ -119,6 +119,13 @@ int cmag_density(const cmag_params* p, float* measures, const float* sortedPos,
                  const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
                  cmag_stream_t stream);
 
+/* in-range neighbours (j != i, |r| < 2h) of every sorted particle, walked
+ * exactly as the density pass walks them.  A measurement aid with no
+ * reference counterpart: it gives the neighbour-pair count per pass. */
+int cmag_count_neighbours(const cmag_params* p, uint32_t* counts, const float* sortedPos,
+                          const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
+                          cmag_stream_t stream);
+
 /* fused force pass scattered to acc[index[i]]:
  *   calcAndApplyAcceleration         fluidSystem.cu:232-272
  *   calculatePoiseuilleAcceleration  poiseuilleFlowSystem.cu:206-247 */
