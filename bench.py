@@ -96,6 +96,92 @@ def measured_traffic(workload_key: str, kernel: str) -> float | None:
         return None
 
 
+def bench_batch(steps: int = 1000, plain_steps: int = 20, tune_cells: bool = True) -> dict:
+    """The batched workload: 256 independent 2-D channels (half Poiseuille, half
+    peristaltic, 1k..64k particles each) stepped together in one launch sequence
+    per step (HIP graphs), against the same instances as 256 separate System
+    objects stepped one after another (the per-instance plain loop)."""
+    import torch
+
+    from cmag_amd import _lib as L
+    from cmag_amd import scenarios as S
+    from cmag_amd.batch import Batch, autotune_cells
+    from cmag_amd.systems import ParamSystem
+
+    def instances(cf):
+        return [(sc.params, sc.pos, sc.vel) for sc in S.channel_batch(cell_factor=cf)]
+
+    t_tune = time.perf_counter()
+    if tune_cells:
+        tuned = autotune_cells(instances, cell_factors=(1.0, 1.3, 2.0), steps=20, steps_per_trial=3)
+    else:
+        tuned = {"cell_factor": 1.0, "tuning": None, "trials": []}
+    t_tune = time.perf_counter() - t_tune
+    scen = S.channel_batch(cell_factor=tuned["cell_factor"])
+    n_total = sum(sc.pos.shape[0] for sc in scen)
+    b = Batch([(sc.params, sc.pos, sc.vel) for sc in scen])
+    if tuned["tuning"]:
+        b.set_tuning(**tuned["tuning"])
+    else:
+        b.autotune(3)
+    b.run(20)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    b.enable_timing(True)
+    b.stage_times()
+    b.run(10, use_graph=False)
+    st = {k: round(v[0] / max(v[1], 1), 4) for k, v in b.stage_times().items()}
+    b.enable_timing(False)
+    finite = all(np.isfinite(b.getArray(k, L.ARRAY_POSITION)).all() for k in range(b.m))
+    tuning = b.tuning
+    b.close()
+
+    systems = [ParamSystem.from_state(sc.params, sc.pos, sc.vel) for sc in scen]
+    for s_ in systems:
+        s_.update()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        for s_ in systems:
+            s_.update()
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    # the same loop with each System replaying its own one-step graph where it can
+    t0 = time.perf_counter()
+    for s_ in systems:
+        s_.run(plain_steps, use_graph=True)
+    torch.cuda.synchronize()
+    plain_graph_wall = time.perf_counter() - t0
+    for s_ in systems:
+        s_.close()
+    batched = n_total * steps / wall
+    plain = n_total * plain_steps / plain_wall
+    plain_graph = n_total * plain_steps / plain_graph_wall
+    return {
+        "workload": (f"256 2-D channels (128 Poiseuille nu~logU(1e-3,1e-1), 128 peristaltic A/H~U(0.1,0.5), "
+                     f"wavelength~U(0.5,2)W), {n_total:,} particles, counts log-uniform 1k..64k"),
+        "num_particles": n_total,
+        "steps": steps,
+        "ms_per_step": 1e3 * wall / steps,
+        "particle_steps_per_s": batched,
+        "stages_ms": st,
+        "tuning": tuning,
+        "cell_factor": tuned["cell_factor"],
+        "autotune_s": round(t_tune, 2),
+        "autotune_trials": [{"cell_factor": t["cell_factor"], "ms_per_step": round(t["ms_per_step"], 3)}
+                            for t in tuned["trials"]],
+        "finite_after": bool(finite),
+        "plain_loop_particle_steps_per_s": plain,
+        "plain_loop_steps": plain_steps,
+        "plain_loop_graph_particle_steps_per_s": plain_graph,
+        "speedup_vs_plain_loop": batched / plain,
+        "speedup_vs_plain_loop_graphs": batched / plain_graph,
+    }
+
+
 def cpu_baseline(target_seconds: float = 12.0) -> dict:
     from cmag_amd import scenarios as S
     from oracle import oracle as O
@@ -137,6 +223,10 @@ def main() -> int:
     ap.add_argument("--morton-xbits", type=int, default=4,
                     help="Z order over bricks of 2^xbits cells along x (0 = plain Morton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-channel comparison")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--workload", choices=["dambreak", "batch"], default="dambreak",
+                    help="batch: only the batched-channel comparison (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -146,6 +236,11 @@ def main() -> int:
         return 2
 
     import torch  # noqa: F401  (initialises the ROCm runtime the library shares)
+
+    if args.workload == "batch":
+        res = bench_batch(steps=args.batch_steps)
+        print(json.dumps({"batched": res}), flush=True)
+        return 0
 
     from cmag_amd import _lib as L
     from cmag_amd import scenarios as S
@@ -235,6 +330,8 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    if not args.no_batch:
+        result["batched"] = bench_batch(steps=args.batch_steps)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)

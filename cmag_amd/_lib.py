@@ -106,7 +106,17 @@ ARRAY_CELLEND = 19
 CELL_EMPTY = 0xFFFFFFFF
 STAGES = ("hash", "sort", "reorder", "density", "force", "integrate")
 
+class BatchTuning(C.Structure):
+    """cmag_batch_tuning (include/cmag_hip.h)"""
+    _fields_ = [("block", C.c_uint32), ("nb_cap", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("steps_per_graph", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 _P = C.POINTER(CmagParams)
+_T = C.POINTER(BatchTuning)
 _vp = C.c_void_p
 _u32 = C.c_uint32
 _i32 = C.c_int32
@@ -165,6 +175,26 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_system_change_right_boundary": (C.c_int, [_vp]),
     "cmag_system_remove_right_boundary": (C.c_int, [_vp]),
     "cmag_system_change_gravity": (C.c_int, [_vp]),
+    "cmag_batch_create": (C.c_int, [_P, C.POINTER(_u32), _u32, C.POINTER(_vp)]),
+    "cmag_batch_destroy": (C.c_int, [_vp]),
+    "cmag_batch_num_instances": (C.c_uint32, [_vp]),
+    "cmag_batch_num_particles": (C.c_uint32, [_vp]),
+    "cmag_batch_num_cells": (C.c_uint32, [_vp]),
+    "cmag_batch_instance_range": (C.c_int, [_vp, _u32, C.POINTER(_u32), C.POINTER(_u32)]),
+    "cmag_batch_get_params": (C.c_int, [_vp, _u32, _P]),
+    "cmag_batch_set_array": (C.c_int, [_vp, _u32, C.c_int, _vp, _u32]),
+    "cmag_batch_get_array": (C.c_int, [_vp, _u32, C.c_int, _vp, _u32]),
+    "cmag_batch_device_ptr": (_vp, [_vp, C.c_int]),
+    "cmag_batch_elapsed_time": (C.c_int, [_vp, _u32, C.POINTER(_f32)]),
+    "cmag_batch_reset_clocks": (C.c_int, [_vp]),
+    "cmag_batch_update": (C.c_int, [_vp]),
+    "cmag_batch_run": (C.c_int, [_vp, _u32, C.c_int]),
+    "cmag_batch_synchronize": (C.c_int, [_vp]),
+    "cmag_batch_get_tuning": (C.c_int, [_vp, _T]),
+    "cmag_batch_set_tuning": (C.c_int, [_vp, _T]),
+    "cmag_batch_autotune": (C.c_int, [_vp, _u32, _T, C.POINTER(_f32)]),
+    "cmag_batch_enable_timing": (C.c_int, [_vp, C.c_int]),
+    "cmag_batch_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]),
 }
 
 # entry points whose int return is a status

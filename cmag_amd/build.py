@@ -32,6 +32,7 @@ SOURCES = [
     "sort.hip",
     "sph.hip",
     "integrate.hip",
+    "batch.hip",
 ]
 
 HIPCC_FLAGS = [

@@ -247,7 +247,8 @@ __device__ __forceinline__ float3 vel_diff_poiseuille(const cmag_params& P, cons
 // terms of the listed neighbours.  Without the split, the ~50-instruction pair
 // term would run for every candidate (some lane of the wave is almost always
 // in range), i.e. ~475 times per particle instead of ~74.  The list holds
-// NB_CAP entries per lane; when a lane fills it, every active lane flushes.
+// CAP entries per lane (NB_CAP unless a caller picks another depth); when a
+// lane fills it, every active lane flushes.
 constexpr int NB_CAP = 16;
 constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
 
@@ -270,8 +271,8 @@ __device__ __forceinline__ float3 image_target(const cmag_params& P, const float
     return q;
 }
 
-// nbl: this wave's neighbour lists in LDS, [NB_CAP][64]
-template <int SYS, int DIM, int HASH, typename Cells>
+// nbl: this wave's neighbour lists in LDS, [CAP][64]
+template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
                                            const float4* __restrict__ sortedPos,
@@ -294,11 +295,16 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     const float av_k = 2.0f * P.alpha * h * P.soundspeed;  // nu * (rho_i + rho_j)
     const float av_eps = 0.001f * h * h;
     const float visc_k = m * (P.mu + P.mu);
+    // pair terms accumulate in f in walk order whenever the list is flushed;
+    // wall terms (DamBreak) go to w as they are met, and f + w is formed once
+    // at the end, so the result does not depend on when flushes happen (list
+    // depth, which particles share the wave)
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
     int count = 0;
 
     auto flush = [&]() {
-        for (int k = 0; k < NB_CAP; ++k) {
+        for (int k = 0; k < CAP; ++k) {
             if (k >= count) break;
             const uint32_t e = nbl[k][lane];
             const uint32_t j = NbEntry::index(e);
@@ -345,22 +351,22 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                 const float ad2 = ad * ad;
                 const float ad6 = ad2 * ad2 * ad2;
                 const float sc = P.lj_D * (ad6 * ad6 - ad6) * frcp(dist * dist);
-                fx += sc * rx; fy += sc * ry; fz += sc * rz;
+                wx += sc * rx; wy += sc * ry; wz += sc * rz;
             } else if (r2 < r2max) {
                 nbl[count][lane] = NbEntry::pack(j, code);
                 ++count;
             }
         }
-        if (__ballot(count == NB_CAP)) flush();
+        if (__ballot(count == CAP)) flush();
     });
     flush();
     const uint32_t k = index[i];
-    if (k < n) acc[k] = make_float4(fx, fy, fz, 0.0f);
+    if (k < n) acc[k] = make_float4(fx + wx, fy + wy, fz + wz, 0.0f);
 }
 
 // Peristalsis: pressure and viscous passes; the fused variant produces both
 // from one neighbour walk (same per-pair arithmetic as the split kernels)
-template <int DIM, int HASH, bool WANT_VISC, bool WANT_PRES, typename Cells>
+template <int DIM, int HASH, bool WANT_VISC, bool WANT_PRES, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void peristalsis_force_body(
     const cmag_params& P, uint32_t i, float4* __restrict__ viscF, float4* __restrict__ presF,
     const float4* __restrict__ measures, const float4* __restrict__ sortedPos,
@@ -384,7 +390,7 @@ __device__ __forceinline__ void peristalsis_force_body(
     int count = 0;
 
     auto flush = [&]() {
-        for (int k = 0; k < NB_CAP; ++k) {
+        for (int k = 0; k < CAP; ++k) {
             if (k >= count) break;
             const uint32_t e = nbl[k][lane];
             const uint32_t j = NbEntry::index(e);
@@ -421,7 +427,7 @@ __device__ __forceinline__ void peristalsis_force_body(
             nbl[count][lane] = NbEntry::pack(j, code);
             ++count;
         }
-        if (__ballot(count == NB_CAP)) flush();
+        if (__ballot(count == CAP)) flush();
     });
     flush();
     const uint32_t k = index[i];

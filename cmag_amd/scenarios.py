@@ -353,3 +353,181 @@ def dambreak3d_particles(target_fluid: int, **kw) -> Scenario:
     (0.4 x 0.8 x 0.4 column, dx = 0.4 / round((target/2)^(1/3)))."""
     n = int(round((target_fluid / 2.0) ** (1.0 / 3.0)))
     return dambreak3d(dx=0.4 / n, **kw)
+
+
+# ---------------------------------------------------------------- channel batches
+def _channel_grid(p: CmagParams, nx: int, dx: float, height: float, h: float, cell_factor: float) -> None:
+    """x-periodic 2-D grid over a channel of nx lattice columns: gx cells span the
+    period nx*dx exactly (gx the largest power of two with cell >= cell_factor*dx),
+    square cells, gy covering `height` plus a margin, one cell along z."""
+    period = nx * dx
+    gx = 1
+    while 2 * gx * cell_factor * dx <= period + 1e-12:
+        gx *= 2
+    cs = period / gx
+    gy = _pow2_at_least(height / cs + 4)
+    p.cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
+    p.grid_size[0], p.grid_size[1], p.grid_size[2] = gx, gy, 1
+    p.cell_size[0] = p.cell_size[1] = p.cell_size[2] = cs
+    p.world_origin[0] = np.float32(-0.5 * period)
+    p.world_origin[1] = np.float32(-0.5 * gy * cs)
+    p.world_origin[2] = np.float32(-0.5 * cs)
+    p.world_size[0] = np.float32(gx) * np.float32(cs)
+    p.world_size[1] = np.float32(gy) * np.float32(cs)
+    p.world_size[2] = np.float32(cs)
+    p.periodic_mask = 1
+
+
+def channel_poiseuille(n_target: int, nu: float, seed: int, width: float = 1.0, bo: int = 3,
+                       vmax: float = 1e-2, h_factor: float = 1.3, aspect: float = 0.5,
+                       jitter: float = 1e-3, cell_factor: float = 1.0) -> Scenario:
+    """One Poiseuille channel of about n_target particles (fluid + walls) in the
+    Poiseuille system's loop bodies: fluid nx x ny with nx ~ aspect*ny, dx =
+    width/ny, bo wall layers (w = 1) above and below, body force 8 nu vmax /
+    width^2, Tait EOS with c0 = 10 vmax, dt = min(0.25 h/c0, 0.125 h^2/nu)."""
+    ny = int(round((-2 * bo + math.sqrt(4 * bo * bo + 4 * n_target / aspect)) / 2))
+    ny = max(8, ny)
+    nx = max(4, int(round(aspect * ny)))
+    dx = width / ny
+    h = h_factor * dx
+    p = _blank(L.SYS_POISEUILLE, 2)
+    p.eos = L.EOS_TAIT
+    _channel_grid(p, nx, dx, (ny + 2 * bo) * dx, h, cell_factor)
+    p.fluid_size[0], p.fluid_size[1], p.fluid_size[2] = nx, ny, 1
+    p.particle_radius = dx / 2.0
+    p.smoothing_radius = h
+    c0 = 10.0 * vmax
+    p.soundspeed = c0
+    p.B = 1000.0 * c0 * c0 / 7.0
+    p.mu = 1000.0 * nu
+    g = 8.0 * nu * vmax / (width * width)
+    p.gravity[0] = g
+    p.delta_time = min(0.25 * h / c0, 0.125 * h * h / nu)
+    p.particle_mass = 1000.0 / lattice_kernel_sum(h, dx, 2)
+    y0 = np.float32(-0.5 * ny * dx)
+    p.wall_bottom = y0
+    p.wall_top = np.float32(y0 + ny * dx)
+    finalize(p)
+    rng = SplitMix64(seed)
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+    nf = nx * ny
+    jit = (rng.uniform(2 * nf).reshape(2, nf) * 2.0 - 1.0) * jitter * dx
+    z = float(p.world_origin[2]) + 0.5 * float(p.cell_size[2])
+    fluid = np.zeros((nf, 4))
+    fluid[:, 0] = float(p.world_origin[0]) + (ii.ravel() + 0.5) * dx + jit[0]
+    fluid[:, 1] = float(y0) + (jj.ravel() + 0.5) * dx + jit[1]
+    fluid[:, 2] = z
+    wi, wj = np.meshgrid(np.arange(nx), np.arange(bo), indexing="xy")
+    wx = float(p.world_origin[0]) + (wi.ravel() + 0.5) * dx
+    walls = np.zeros((2 * nx * bo, 4))
+    walls[:nx * bo, 0] = wx
+    walls[:nx * bo, 1] = float(y0) - (wj.ravel() + 0.5) * dx
+    walls[nx * bo:, 0] = wx
+    walls[nx * bo:, 1] = float(y0) + ny * dx + (wj.ravel() + 0.5) * dx
+    walls[:, 2] = z
+    walls[:, 3] = 1.0
+    pos = np.concatenate([fluid, walls]).astype(np.float32)
+    meta = dict(kind="poiseuille", dx=dx, h=h, nu=nu, vmax=vmax, g=g, nx=nx, ny=ny, width=width)
+    return Scenario(p, pos, np.zeros_like(pos), nf, meta)
+
+
+def channel_peristalsis(n_target: int, amp_ratio: float, wl_ratio: float, seed: int, width: float = 1.0,
+                        bo: int = 3, wave_speed: float = 1e-2, nu: float = 1e-2, h_factor: float = 1.5,
+                        jitter: float = 1e-3, cell_factor: float = 1.0) -> Scenario:
+    """One peristaltic channel of about n_target particles in the Peristalsis
+    system's loop bodies (PeristalsisDensity.cu, PeristalsisPressureForce.cu,
+    PeristalsisViscousForce.cu, PeristalsisIntegrate.cu): half-width H =
+    width/2 = ny*r, walls w = +-2, +-4, +-6 on the sinusoid
+    y = -+(H - A sin(k(x - x0) - k c t)) -+ layer offset with A = amp_ratio*H,
+    wavelength = domain length = wl_ratio*width (k = pi/(nx r) as in GetWave,
+    PeristalsisKernel.cuh:59-62), wave speed c.  The walls start on the
+    sinusoid (no configuration ramp) and carry the wall velocity; the fluid is
+    the lattice inside the channel at t = 0.  EOS p = 50 c_s rho with 50 c_s =
+    (10 max(c, A k c))^2, mu = 1000 nu, no body force."""
+    ny = int(round((-2 * bo + math.sqrt(4 * bo * bo + 4 * n_target / wl_ratio)) / 2))
+    ny = max(8, ny)
+    nx = max(4, int(round(wl_ratio * ny)))
+    dx = width / ny
+    r = dx / 2.0
+    h = h_factor * dx
+    H = ny * r
+    A = amp_ratio * H
+    p = _blank(L.SYS_PERISTALSIS, 2)
+    p.eos = L.EOS_LINEAR_50C
+    _channel_grid(p, nx, dx, 2 * (H + A) + 2 * bo * dx, h, cell_factor)
+    p.fluid_size[0], p.fluid_size[1], p.fluid_size[2] = nx, ny, 1
+    p.boundary_offset = bo
+    p.particle_radius = r
+    p.smoothing_radius = h
+    p.amplitude = A
+    p.wave_speed = wave_speed
+    p.wave_k = np.float32(PI_F / np.float32(nx * np.float32(r)))
+    k = 2.0 * math.pi / (nx * dx)
+    c_eff = 10.0 * max(wave_speed, A * k * wave_speed)
+    p.soundspeed = c_eff * c_eff / 50.0
+    p.mu = 1000.0 * nu
+    p.delta_time = min(0.25 * h / c_eff, 0.125 * h * h / nu)
+    p.particle_mass = 1000.0 / lattice_kernel_sum(h, dx, 2)
+    p.is_boundary_configuration = 0
+    p.wall_velocity = 1
+    finalize(p)
+    ox = np.float32(p.world_origin[0])
+    wk = np.float32(p.wave_k)
+    rng = SplitMix64(seed)
+    # fluid lattice over |y| < H + A, kept inside the channel at t = 0
+    rows = int(math.ceil(2 * (H + A) / dx))
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(rows), indexing="xy")
+    x = float(ox) + (ii.ravel() + 0.5) * dx
+    y = -(H + A) + (jj.ravel() + 0.5) * dx
+    s = np.sin(k * (x - float(ox)))
+    keep = (y >= -H + A * s + 0.5 * dx) & (y <= H - A * s - 0.5 * dx)
+    x, y = x[keep], y[keep]
+    nf = x.size
+    jit = (rng.uniform(2 * nf).reshape(2, nf) * 2.0 - 1.0) * jitter * dx
+    z = float(p.world_origin[2]) + 0.5 * float(p.cell_size[2])
+    fluid = np.zeros((nf, 4))
+    fluid[:, 0] = x + jit[0]
+    fluid[:, 1] = y + jit[1]
+    fluid[:, 2] = z
+    # walls in float32 with the kernels' formula (PeristalsisKernel.cu:144-171)
+    wx = (ox + np.float32(r) * (2 * np.arange(nx, dtype=np.float32) + 1)).astype(np.float32)
+    wave = np.sin((wk * (wx - ox)).astype(np.float32)).astype(np.float32)
+    nyr = np.float32(ny) * np.float32(r)
+    walls = []
+    for layer in range(bo):
+        wcode = np.float32(2.0 * (1 + layer))
+        yb = -nyr + np.float32(A) * wave - np.float32(r) * (wcode - 1)
+        yt = nyr - np.float32(A) * wave + np.float32(r) * (wcode - 1)
+        for yy, code in ((yb, wcode), (yt, -wcode)):
+            wl = np.zeros((nx, 4), np.float32)
+            wl[:, 0] = wx
+            wl[:, 1] = yy
+            wl[:, 2] = z
+            wl[:, 3] = code
+            walls.append(wl)
+    pos = np.concatenate([fluid.astype(np.float32)] + walls).astype(np.float32)
+    meta = dict(kind="peristalsis", dx=dx, h=h, nu=nu, amplitude=A, wavelength=nx * dx, nx=nx, ny=ny,
+                amp_ratio=amp_ratio, wl_ratio=wl_ratio, width=width)
+    return Scenario(p, pos, np.zeros_like(pos), nf, meta)
+
+
+def channel_batch(m: int = 256, seed: int = 20240, n_lo: int = 1024, n_hi: int = 65536,
+                  nu_lo: float = 1e-3, nu_hi: float = 1e-1, cell_factor: float = 1.0) -> list[Scenario]:
+    """The batched workload: m independent 2-D channels with particle counts
+    log-uniform in [n_lo, n_hi]; the first half Poiseuille with nu log-uniform in
+    [nu_lo, nu_hi], the second half peristaltic with amplitude ratio U(0.1, 0.5)
+    and wavelength U(0.5, 2) x width.  Seeded (splitmix64)."""
+    rng = SplitMix64(seed)
+    out = []
+    ln = math.log
+    for k in range(m):
+        u = rng.uniform(4)
+        n = int(round(math.exp(ln(n_lo) + float(u[0]) * (ln(n_hi) - ln(n_lo)))))
+        if k < m // 2:
+            nu = math.exp(ln(nu_lo) + float(u[1]) * (ln(nu_hi) - ln(nu_lo)))
+            out.append(channel_poiseuille(n, nu, seed=seed + k, cell_factor=cell_factor))
+        else:
+            amp = 0.1 + 0.4 * float(u[2])
+            wl = 0.5 + 1.5 * float(u[3])
+            out.append(channel_peristalsis(n, amp, wl, seed=seed + k, cell_factor=cell_factor))
+    return out

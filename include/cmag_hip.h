@@ -230,6 +230,59 @@ int cmag_system_change_right_boundary(cmag_system* s);
 int cmag_system_remove_right_boundary(cmag_system* s);
 int cmag_system_change_gravity(cmag_system* s);
 
+/* ---- batches: many independent systems in one step ------------------------
+ * No reference counterpart: the reference's process-global __constant__
+ * parameters (fluid_kernel.cu:17, fluidSystem.cu:17-19) allow one system per
+ * process.  A batch holds m instances (any system kind; one dimension and hash
+ * order per batch) with their own parameter blocks and particle counts in one
+ * arena; cmag_batch_update steps every instance once, with the same per-particle
+ * arithmetic as cmag_system_update on each instance alone (bit for bit).
+ * Particles of instance k are numbered [first, first + count) of the arena
+ * (cmag_batch_instance_range); set/get_array move one instance's float4 rows. */
+typedef struct cmag_batch cmag_batch;
+
+/* launch shape picked by cmag_batch_autotune (or set by hand):
+ *   block            threads per workgroup and particles per tile: 64, 128, 256
+ *   nb_cap           per-lane neighbour-list depth of the force pass (LDS): 8, 16, 32
+ *   blocks_per_cu    persistent workgroups per CU, each running a contiguous
+ *                    chunk of tiles; 0 = one workgroup per tile
+ *   steps_per_graph  steps captured per HIP graph by cmag_batch_run */
+typedef struct cmag_batch_tuning {
+    uint32_t block;
+    uint32_t nb_cap;
+    uint32_t blocks_per_cu;
+    uint32_t steps_per_graph;
+} cmag_batch_tuning;
+
+int cmag_batch_create(const cmag_params* params, const uint32_t* counts, uint32_t m, cmag_batch** out);
+int cmag_batch_destroy(cmag_batch* b);
+uint32_t cmag_batch_num_instances(const cmag_batch* b);
+uint32_t cmag_batch_num_particles(const cmag_batch* b);
+uint32_t cmag_batch_num_cells(const cmag_batch* b);
+int cmag_batch_instance_range(const cmag_batch* b, uint32_t inst, uint32_t* first, uint32_t* count);
+int cmag_batch_get_params(const cmag_batch* b, uint32_t inst, cmag_params* out);
+/* count must equal the instance's particle count */
+int cmag_batch_set_array(cmag_batch* b, uint32_t inst, int array, const float* data, uint32_t count);
+int cmag_batch_get_array(cmag_batch* b, uint32_t inst, int array, float* data, uint32_t count);
+void* cmag_batch_device_ptr(cmag_batch* b, int array);
+/* elapsed time of one instance (kept on the device, read back) */
+int cmag_batch_elapsed_time(cmag_batch* b, uint32_t inst, float* out);
+/* elapsed time 0 and a fresh Peristalsis boundary-configuration state */
+int cmag_batch_reset_clocks(cmag_batch* b);
+/* one step of every instance: update() of each System, in one launch sequence */
+int cmag_batch_update(cmag_batch* b);
+/* steps steps; use_graph replays HIP graphs of steps_per_graph steps */
+int cmag_batch_run(cmag_batch* b, uint32_t steps, int use_graph);
+int cmag_batch_synchronize(cmag_batch* b);
+int cmag_batch_get_tuning(const cmag_batch* b, cmag_batch_tuning* out);
+int cmag_batch_set_tuning(cmag_batch* b, const cmag_batch_tuning* t);
+/* times steps_per_trial steps of every launch shape on the batch's own state,
+ * puts the state back after each trial, and keeps the fastest shape */
+int cmag_batch_autotune(cmag_batch* b, uint32_t steps_per_trial, cmag_batch_tuning* best,
+                        float* best_ms_per_step);
+int cmag_batch_enable_timing(cmag_batch* b, int on);
+int cmag_batch_stage_times(cmag_batch* b, double* ms, uint32_t* counts, int max_stages);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,3 +146,45 @@ def test_splitmix64_reference_value():
     assert g.next_u64() == 0xE220A8397B1DCDAF
     u = S.SplitMix64(1973).uniform(1000)
     assert np.all((u >= 0) & (u < 1)) and abs(u.mean() - 0.5) < 0.05
+
+
+def test_channel_batch_workload_geometry():
+    """the batched workload: 256 channels, counts log-uniform in [1k, 64k],
+    half Poiseuille / half peristaltic, each grid spanning its x-period exactly"""
+    import math
+
+    import numpy as np
+
+    from cmag_amd import scenarios as S
+
+    b = S.channel_batch()
+    assert len(b) == 256
+    kinds = [s.meta["kind"] for s in b]
+    assert kinds[:128] == ["poiseuille"] * 128 and kinds[128:] == ["peristalsis"] * 128
+    counts = np.array([s.pos.shape[0] for s in b])
+    assert counts.min() >= 900 and counts.max() <= 70000
+    assert 3e6 < counts.sum() < 5e6
+    nus = np.array([s.meta["nu"] for s in b[:128]])
+    assert nus.min() >= 1e-3 and nus.max() <= 1e-1
+    for s in b:
+        p = s.params
+        dx = s.meta["dx"]
+        period = s.meta["nx"] * dx
+        assert abs(p.grid_size[0] * p.cell_size[0] - period) < 1e-6 * period
+        # cell count from the exact (double) h and cell, as the reference sizes
+        # h = 3r with cell = 2r: cellcount 3
+        assert p.cellcount == math.ceil(2 * s.meta["h"] / (period / p.grid_size[0]) - 1e-9)
+        assert p.grid_size[2] == 1 and p.dim == 2 and p.periodic_mask == 1
+        inside = (s.pos[:, 1] - p.world_origin[1]) / p.cell_size[1]
+        assert inside.min() > 0 and inside.max() < p.grid_size[1]
+    for s in b[128:]:
+        p = s.params
+        assert 0.1 <= s.meta["amp_ratio"] <= 0.5 and 0.5 <= s.meta["wl_ratio"] <= 2.0
+        # fluid strictly between the innermost wall layers at t = 0
+        fl = s.pos[:, 3] == 0
+        x, y = s.pos[fl, 0].astype(np.float64), s.pos[fl, 1].astype(np.float64)
+        k = 2 * math.pi / s.meta["wavelength"]
+        sn = np.sin(k * (x - float(p.world_origin[0])))
+        H, A = p.fluid_size[1] * p.particle_radius, float(p.amplitude)
+        assert np.all(y > -H + A * sn) and np.all(y < H - A * sn)
+        assert np.all(np.isin(s.pos[~fl, 3], [2, 4, 6, -2, -4, -6]))

@@ -1,0 +1,138 @@
+"""Batched stepping on the GPU: every instance of a batch must step exactly as a
+single System built from the same parameter block and state (bit for bit), on
+the eager and the graph path, across the Peristalsis boundary-configuration
+switch, and the autotuner must leave the state as it found it."""
+import numpy as np
+import pytest
+
+from cmag_amd import _lib as L
+from cmag_amd import scenarios as S
+from cmag_amd.batch import Batch
+from cmag_amd.systems import ParamSystem
+
+pytestmark = pytest.mark.gpu
+
+ARRAYS = (L.ARRAY_POSITION, L.ARRAY_VELOCITY, L.ARRAY_VELOCITYLEAPFROG)
+
+
+def mixed_instances():
+    """three kinds, uneven sizes; one Peristalsis channel still in its
+    boundary-configuration phase, with a ramp short enough to finish"""
+    rng = np.random.default_rng(5)
+    out = []
+    for k, n in enumerate((1500, 4200, 2600)):
+        sc = S.channel_poiseuille(n, nu=[2e-3, 2e-2, 8e-2][k], seed=100 + k)
+        vel = np.zeros_like(sc.pos)
+        vel[:sc.num_fluid, :2] = rng.normal(scale=5e-3, size=(sc.num_fluid, 2))
+        out.append((sc.params, sc.pos, vel))
+    for k, (n, a, wl) in enumerate(((3300, 0.2, 0.7), (1800, 0.45, 1.8))):
+        sc = S.channel_peristalsis(n, a, wl, seed=200 + k)
+        out.append((sc.params, sc.pos, None))
+    sc = S.channel_peristalsis(2000, 0.3, 1.0, seed=300)
+    p = sc.params.copy()
+    p.is_boundary_configuration = 1
+    p.amplitude = np.float32(3.5 * p.delta_time * 1e-3)  # ramp of 4 steps, then 1000 dt of relaxation
+    out.append((p, sc.pos, None))
+    # a reference DamBreak block (2-D, linear hash) shares the batch
+    p = S.params_dambreak((24, 20, 1), 3, (64, 32, 4), 1 / 128)
+    from oracle import oracle as O
+    n = O.dambreak_count((24, 20, 1), 3, (64, 32, 4))
+    out.append((p, O.dambreak_init(p, n), None))
+    return out
+
+
+def singles(instances):
+    out = []
+    for p, pos, vel in instances:
+        out.append(ParamSystem.from_state(p, pos, vel))
+    return out
+
+
+def assert_same(b, sys_list):
+    for k, s in enumerate(sys_list):
+        for a in ARRAYS:
+            got, want = b.getArray(k, a), s.getArray(a)
+            assert np.array_equal(got, want), (k, a, np.abs(got - want).max())
+        assert b.elapsed(k) == s.getElapsedTime()
+
+
+def test_batch_equals_single_systems_eager(device):
+    inst = mixed_instances()
+    b = Batch(inst)
+    ss = singles(inst)
+    for _ in range(12):
+        b.update()
+        for s in ss:
+            s.update()
+    assert_same(b, ss)
+
+
+def test_batch_graph_and_launch_shapes(device):
+    inst = mixed_instances()
+    b = Batch(inst)
+    ss = singles(inst)
+    b.set_tuning(block=128, nb_cap=8, blocks_per_cu=2, steps_per_graph=4)
+    b.run(10, use_graph=True)          # two graph launches + two eager steps
+    for s in ss:
+        for _ in range(10):
+            s.update()
+    assert_same(b, ss)
+    b.set_tuning(block=64, nb_cap=32, blocks_per_cu=0, steps_per_graph=3)
+    b.run(3, use_graph=True)
+    b.update()                          # eager work between replays
+    b.run(6, use_graph=True)
+    for s in ss:
+        for _ in range(10):
+            s.update()
+    assert_same(b, ss)
+
+
+def test_configuration_switch_matches_single_system(device):
+    """a Peristalsis channel through its whole boundary-configuration phase
+    (ramp, 1000 dt of relaxation, switch to the moving wave)"""
+    sc = S.channel_peristalsis(1500, 0.3, 1.0, seed=301)
+    p = sc.params.copy()
+    p.is_boundary_configuration = 1
+    p.amplitude = np.float32(2.5 * p.delta_time * 1e-3)
+    b = Batch([(p, sc.pos, None)])
+    s = ParamSystem.from_state(p, sc.pos, None)
+    b.run(1010, use_graph=True)
+    for _ in range(1010):
+        s.update()
+    assert_same(b, [s])
+
+
+def test_autotune_restores_state(device):
+    inst = mixed_instances()
+    b = Batch(inst)
+    b.run(3, use_graph=False)
+    before = [[b.getArray(k, a) for a in ARRAYS] for k in range(b.m)]
+    t_before = [b.elapsed(k) for k in range(b.m)]
+    tune, ms = b.autotune(steps_per_trial=2)
+    assert tune["block"] in (64, 128, 256) and tune["nb_cap"] in (8, 16, 32) and ms > 0
+    assert b.tuning == tune
+    for k in range(b.m):
+        for a, want in zip(ARRAYS, before[k]):
+            assert np.array_equal(b.getArray(k, a), want)
+        assert b.elapsed(k) == t_before[k]
+    # and stepping continues as a fresh batch would
+    ref = Batch(inst)
+    ref.run(3, use_graph=False)
+    b.run(5)
+    ref.run(5, use_graph=False)
+    for k in range(b.m):
+        assert np.array_equal(b.getArray(k, L.ARRAY_POSITION), ref.getArray(k, L.ARRAY_POSITION))
+
+
+def test_batch_errors(device):
+    sc2 = S.channel_poiseuille(1200, nu=1e-2, seed=1)
+    sc3 = S.dambreak3d(dx=0.05)
+    with pytest.raises(L.CmagError, match="dimension"):
+        Batch([(sc2.params, sc2.pos, None), (sc3.params, sc3.pos, None)])
+    b = Batch([(sc2.params, sc2.pos, None)])
+    with pytest.raises(L.CmagError, match="count"):
+        b.setArray(0, L.ARRAY_POSITION, sc2.pos[:-1])
+    with pytest.raises(L.CmagError, match="instance"):
+        L.call("cmag_batch_get_array", b._h, 5, L.ARRAY_POSITION, None, 0)
+    with pytest.raises(L.CmagError, match="block"):
+        b.set_tuning(block=96)

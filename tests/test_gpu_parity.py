@@ -87,12 +87,23 @@ def make_case(name):
         p, pos = sc.params, sc.pos.copy()
         vel = np.zeros_like(pos)
         vel[:sc.num_fluid, :3] = rng.normal(scale=0.1, size=(sc.num_fluid, 3))
+    elif name == "batch_poiseuille":
+        sc = S.channel_poiseuille(3000, nu=2e-2, seed=11)
+        p, pos = sc.params, sc.pos.copy()
+        vel = np.zeros_like(pos)
+        vel[:sc.num_fluid, :2] = rng.normal(scale=1e-2, size=(sc.num_fluid, 2))
+    elif name == "batch_peristalsis":
+        sc = S.channel_peristalsis(3000, 0.3, 1.2, seed=12)
+        p, pos = sc.params, sc.pos.copy()
+        vel = np.zeros_like(pos)
+        vel[:sc.num_fluid, :2] = rng.normal(scale=1e-2, size=(sc.num_fluid, 2))
     else:
         raise KeyError(name)
     return p, pos.astype(np.float32), vel.astype(np.float32)
 
 
-CASES = ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_channel", "cfg_dam3d", "cfg_dam3d_morton"]
+CASES = ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_channel", "cfg_dam3d", "cfg_dam3d_morton",
+         "batch_poiseuille", "batch_peristalsis"]
 
 
 # ---------------------------------------------------------------- hash
@@ -197,7 +208,7 @@ def test_density_parity(device, oracle, case):
 
 
 # ---------------------------------------------------------------- forces
-@pytest.mark.parametrize("case", [c for c in CASES if c != "ref_peristalsis"])
+@pytest.mark.parametrize("case", [c for c in CASES if not c.endswith("peristalsis")])
 def test_force_parity(device, oracle, case):
     Stages, normwise = _gpu()
     p, pos, vel = make_case(case)
@@ -216,9 +227,10 @@ def test_force_parity(device, oracle, case):
     assert normwise(a[:, :3], want64[:, :3]) <= 1e-4
 
 
-def test_peristalsis_pressure_and_viscous_parity(device, oracle):
+@pytest.mark.parametrize("case", ["ref_peristalsis", "batch_peristalsis"])
+def test_peristalsis_pressure_and_viscous_parity(device, oracle, case):
     Stages, normwise = _gpu()
-    p, pos, vel = make_case("ref_peristalsis")
+    p, pos, vel = make_case(case)
     ref = oracle.run_pipeline(p, pos, vel)
     m, sp, sv, idx, cs, ce = (ref["measures"], ref["sorted_pos"], ref["sorted_vel"], ref["index"],
                               ref["cell_start"], ref["cell_end"])
@@ -229,7 +241,8 @@ def test_peristalsis_pressure_and_viscous_parity(device, oracle):
 
 
 # ---------------------------------------------------------------- integrate / walls
-@pytest.mark.parametrize("case", ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_dam3d"])
+@pytest.mark.parametrize("case", ["ref_dambreak", "ref_poiseuille", "ref_peristalsis", "cfg_dam3d",
+                                  "batch_peristalsis"])
 def test_integrate_bitexact(device, oracle, case):
     Stages, _ = _gpu()
     p, pos, vel = make_case(case)
