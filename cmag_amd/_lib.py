@@ -175,6 +175,11 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_system_change_right_boundary": (C.c_int, [_vp]),
     "cmag_system_remove_right_boundary": (C.c_int, [_vp]),
     "cmag_system_change_gravity": (C.c_int, [_vp]),
+    "cmag_integrate_vjp": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "cmag_force_vjp": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "cmag_density_vjp": (C.c_int, [_P, _vp, _vp, _vp, _vp, _vp, _u32, _vp]),
+    "cmag_system_vjp_step": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "cmag_system_copy_array_device": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int]),
     "cmag_batch_create": (C.c_int, [_P, C.POINTER(_u32), _u32, C.POINTER(_vp)]),
     "cmag_batch_destroy": (C.c_int, [_vp]),
     "cmag_batch_num_instances": (C.c_uint32, [_vp]),

@@ -33,6 +33,7 @@ SOURCES = [
     "sph.hip",
     "integrate.hip",
     "batch.hip",
+    "adjoint.hip",
 ]
 
 HIPCC_FLAGS = [

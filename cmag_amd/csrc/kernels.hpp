@@ -62,4 +62,16 @@ int launch_configure_boundary(const cmag_params& P, float* pos, float wave_heigh
 int launch_shift_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
 int launch_remove_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
 
+// adjoints of the Poiseuille step (adjoint.hip)
+int launch_integrate_vjp(const cmag_params& P, const float* pos, float* gpos, float* gvel, float* gvelLF,
+                         float* gacc, double* ggrav, uint32_t n, hipStream_t s);
+int launch_gather_f4(float* dst, const float* src, const uint32_t* index, uint32_t n, hipStream_t s);
+int launch_force_vjp(const cmag_params& P, float* gposS, float* gvelS, float* grho, float* gmu, double* gmu_acc,
+                     const float* gaccS, const float* measures, const float* sortedPos, const float* sortedVel,
+                     const uint32_t* cellStart, const uint32_t* cellEnd, const uint32_t* cellFirst, uint32_t n,
+                     hipStream_t s);
+int launch_density_vjp(const cmag_params& P, float* gposS, const float* grho, const float* sortedPos,
+                       const uint32_t* cellStart, const uint32_t* cellEnd, const uint32_t* cellFirst, uint32_t n,
+                       const uint32_t* index, const float* gvelS, float* gpos, float* gvelLF, hipStream_t s);
+
 }  // namespace cmag

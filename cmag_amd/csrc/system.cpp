@@ -33,6 +33,8 @@ struct cmag_system {
     uint32_t *cellFirst = nullptr, *scanWs = nullptr;  // compact cell offsets for the walkers
     void* sortWs = nullptr;
     size_t sortWsBytes = 0;
+    // adjoint scratch (sorted order), allocated by the first vjp_step
+    float *gaccU = nullptr, *gaccS = nullptr, *gposS = nullptr, *gvelS = nullptr, *grho = nullptr;
 
     float elapsed = 0.0f;
     // Peristalsis boundary-configuration state, PeristalsisSystem.cpp:330-343
@@ -72,7 +74,8 @@ struct cmag_system {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
-                        hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs};
+                        hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
+                        gaccU, gaccS, gposS, gvelS, grho};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -109,8 +112,9 @@ struct cmag_system {
 
     int passes() const { return (key_bits + 7) / 8; }
 
-    // one update(): hash -> sort -> reorder/cellStart -> density -> forces -> integrate
-    int enqueue_step(float wave_time) {
+    // hash -> sort -> reorder/cellStart -> density: the neighbour structure
+    // and densities of the current state
+    int enqueue_front() {
         // the hash goes to whichever buffer makes the sort end in (hash, index)
         const bool odd = passes() % 2 == 1;
         uint32_t* hk = odd ? hashAlt : hash;
@@ -135,6 +139,12 @@ struct cmag_system {
         CMAG_TRY(launch_cell_offsets(cellStart, cellEnd, cellFirst, numCells, n, scanWs, stream));
         CMAG_TRY(mark(CMAG_STAGE_DENSITY));
         CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
+        return CMAG_OK;
+    }
+
+    // one update(): hash -> sort -> reorder/cellStart -> density -> forces -> integrate
+    int enqueue_step(float wave_time) {
+        CMAG_TRY(enqueue_front());
         CMAG_TRY(mark(CMAG_STAGE_FORCE));
         if (P.system == CMAG_SYS_PERISTALSIS) {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
@@ -216,6 +226,49 @@ struct cmag_system {
             CMAG_HIP_TRY(hipGraphLaunch(graph, stream));
             elapsed += P.delta_time;
         }
+        return CMAG_OK;
+    }
+
+    // adjoint of one update() at the current state (pos, vel, velLF), which
+    // is left unchanged: (gpos, gvel, gvelLF) hold the adjoints of the step's
+    // outputs on entry and of its inputs on return; gmu_acc[n] and
+    // ggrav_acc[4n] (FP64, optional) accumulate the parameter adjoints
+    int vjp_step(float* gpos, float* gvel, float* gvelLF, double* gmu_acc, double* ggrav_acc) {
+        if (P.system != CMAG_SYS_POISEUILLE)
+            return fail(CMAG_E_INVALID, "vjp_step: the adjoint covers the Poiseuille system");
+        if (n && (!gpos || !gvel || !gvelLF)) return fail(CMAG_E_INVALID, "vjp_step: null adjoint");
+        if (!gaccU) {
+            const size_t f4 = sizeof(float) * 4 * (size_t)(n ? n : 1);
+            float** fb[] = {&gaccU, &gaccS, &gposS, &gvelS};
+            for (float** b : fb) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
+            CMAG_HIP_TRY(hipMalloc((void**)&grho, sizeof(float) * (size_t)(n ? n : 1)));
+        }
+        invalidate_graph();
+        const bool was_timing = timing;
+        timing = false;
+        int rc = enqueue_front();
+        timing = was_timing;
+        CMAG_TRY(rc);
+        CMAG_TRY(launch_integrate_vjp(P, pos, gpos, gvel, gvelLF, gaccU, ggrav_acc, n, stream));
+        CMAG_TRY(launch_gather_f4(gaccS, gaccU, index, n, stream));
+        CMAG_TRY(launch_force_vjp(P, gposS, gvelS, grho, nullptr, gmu_acc, gaccS, measures, sortedPos, sortedVel,
+                                  cellStart, cellEnd, cellFirst, n, stream));
+        CMAG_TRY(launch_density_vjp(P, gposS, grho, sortedPos, cellStart, cellEnd, cellFirst, n, index, gvelS,
+                                    gpos, gvelLF, stream));
+        return CMAG_OK;
+    }
+
+    // device-to-device copies, ordered on the system's stream
+    int copy_array_device(int a, float* ext, int start, int count, bool to_system) {
+        float* d = farray(a);
+        if (!d) return fail(CMAG_E_INVALID, "array copy: unknown array for this system");
+        if (start < 0 || count < 0 || (uint64_t)start + (uint64_t)count > n)
+            return fail(CMAG_E_INVALID, "array copy: range outside the particle arrays");
+        if (count == 0) return CMAG_OK;
+        if (!ext) return fail(CMAG_E_INVALID, "array copy: null device pointer");
+        float* sys = d + 4 * (size_t)start;
+        CMAG_HIP_TRY(hipMemcpyAsync(to_system ? sys : ext, to_system ? ext : sys,
+                                    sizeof(float) * 4 * (size_t)count, hipMemcpyDeviceToDevice, stream));
         return CMAG_OK;
     }
 
@@ -482,6 +535,19 @@ int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream) {
     s->stream = stream ? as_stream(stream) : s->own_stream;
     s->invalidate_graph();
     return CMAG_OK;
+}
+
+int cmag_system_vjp_step(cmag_system* s, float* gpos, float* gvel, float* gvelLF, double* gmu_acc,
+                          double* ggrav_acc) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_TRY(s->collect());
+    return s->vjp_step(gpos, gvel, gvelLF, gmu_acc, ggrav_acc);
+}
+
+int cmag_system_copy_array_device(cmag_system* s, int array, float* device_data, int start, int count,
+                                  int to_system) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    return s->copy_array_device(array, device_data, start, count, to_system != 0);
 }
 
 int cmag_system_synchronize(cmag_system* s) {

@@ -531,3 +531,26 @@ def channel_batch(m: int = 256, seed: int = 20240, n_lo: int = 1024, n_hi: int =
             wl = 0.5 + 1.5 * float(u[3])
             out.append(channel_peristalsis(n, amp, wl, seed=seed + k, cell_factor=cell_factor))
     return out
+
+
+# ---------------------------------------------------------------- adjoint workload
+def adjoint_channel(seed: int = 1973, nu_star: float = 2e-2, nu0: float = 1e-2, g0_factor: float = 1.3,
+                    vmax: float = 1e-2, nx: int = 128, ny: int = 128, steps: int = 200,
+                    checkpoint_every: int = 20, bins: int = 64) -> dict:
+    """The differentiable workload: a 2-D Poiseuille channel of nx x ny fluid
+    particles (dx = 1/ny, x-period nx*dx), 3 wall layers, cfg-1 physics (Tait,
+    c0 = 10 vmax, h = 1.3 dx).  The target profile comes from (nu*, g*) with
+    g* = 8 nu* vmax (0.5 + U) (seeded); the gradient is taken at (nu0, g0 =
+    g0_factor g*).  The time step is fixed for both, from the larger
+    viscosity.  Returns the scenario, the two parameter blocks, and the run
+    lengths."""
+    rng = SplitMix64(seed)
+    g_star = 8.0 * nu_star * vmax * (0.5 + float(rng.uniform(1)[0]))
+    sc = poiseuille_channel(nx=nx, ny=ny, nu=max(nu_star, nu0), vmax=vmax, gravity=g_star, seed=seed)
+    p_star = sc.params.copy()
+    p_star.mu = 1000.0 * nu_star
+    p0 = sc.params.copy()
+    p0.mu = 1000.0 * nu0
+    p0.gravity[0] = g0_factor * g_star
+    return dict(scenario=sc, params_star=p_star, params0=p0, g_star=g_star, nu_star=nu_star, nu0=nu0,
+                steps=steps, checkpoint_every=checkpoint_every, bins=bins)

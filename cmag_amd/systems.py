@@ -128,6 +128,22 @@ class SystemBase:
     def synchronize(self) -> None:
         call("cmag_system_synchronize", self._h)
 
+    # device-side state access (stream-ordered on the system's stream)
+    def copy_from_device(self, array: int, device_ptr: int, count: int | None = None) -> None:
+        """rows [0, count) of `array` <- device memory at device_ptr"""
+        n = self.getNumParticles() if count is None else count
+        call("cmag_system_copy_array_device", self._h, array, device_ptr, 0, n, 1)
+
+    def copy_to_device(self, array: int, device_ptr: int, count: int | None = None) -> None:
+        """device memory at device_ptr <- rows [0, count) of `array`"""
+        n = self.getNumParticles() if count is None else count
+        call("cmag_system_copy_array_device", self._h, array, device_ptr, 0, n, 0)
+
+    def vjp_step(self, gpos: int, gvel: int, gvel_lf: int, gmu_acc: int = 0, ggrav_acc: int = 0) -> None:
+        """adjoint of one update() at the current state (device pointers; see
+        cmag_system_vjp_step)"""
+        call("cmag_system_vjp_step", self._h, gpos, gvel, gvel_lf, gmu_acc or None, ggrav_acc or None)
+
     def set_stream(self, stream: int) -> None:
         call("cmag_system_set_stream", self._h, stream)
 

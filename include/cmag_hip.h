@@ -230,6 +230,42 @@ int cmag_system_change_right_boundary(cmag_system* s);
 int cmag_system_remove_right_boundary(cmag_system* s);
 int cmag_system_change_gravity(cmag_system* s);
 
+/* ---- adjoint of the Poiseuille step (no reference counterpart) -------------
+ * Vector-Jacobian products of this library's forward passes with the step's
+ * neighbour structure held fixed (it is piecewise constant); walls (w == 1)
+ * are not state.  They differentiate
+ *   density + EOS     calculatePoiseuilleDensity, poiseuilleFlowSystem.cu:167-204
+ *   force             calculatePoiseuilleAcceleration, poiseuilleFlowSystem.cu:206-247
+ *   integrate         integratePoiseuilleSystem, poiseuilleFlowSystem.cu:84-101
+ * Stage launchers (sorted order unless noted, float4 rows):
+ *   cmag_integrate_vjp  unsorted; gpos/gvel/gvelLF: adjoints of the step's
+ *                       outputs in, of its inputs out; gacc out; ggrav (FP64,
+ *                       4 per particle, optional) += body-force adjoint shares
+ *   cmag_force_vjp      from gaccS: gposS, gvelS (leap-frog velocity), grho
+ *                       (one float per particle), gmu (optional, one float per
+ *                       particle: the target's share of d/d mu), all written
+ *   cmag_density_vjp    gposS += the density pass's share for grho */
+int cmag_integrate_vjp(const cmag_params* p, const float* pos, float* gpos, float* gvel, float* gvelLF,
+                       float* gacc, double* ggrav, uint32_t n, cmag_stream_t stream);
+int cmag_force_vjp(const cmag_params* p, float* gposS, float* gvelS, float* grho, float* gmu, const float* gaccS,
+                   const float* measures, const float* sortedPos, const float* sortedVel,
+                   const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, cmag_stream_t stream);
+int cmag_density_vjp(const cmag_params* p, float* gposS, const float* grho, const float* sortedPos,
+                     const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, cmag_stream_t stream);
+/* adjoint of one update() at the system's current state, which it leaves
+ * unchanged (the neighbour structure and densities are recomputed):
+ * gpos/gvel/gvelLF (device, unsorted, n float4 each) hold the adjoints of the
+ * step's outputs on entry and of its inputs on return; gmu_acc (n doubles)
+ * and ggrav_acc (4n doubles), both optional, accumulate the adjoints of mu
+ * and of the body force per particle (sum them for the totals) */
+int cmag_system_vjp_step(cmag_system* s, float* gpos, float* gvel, float* gvelLF, double* gmu_acc,
+                         double* ggrav_acc);
+/* device-to-device copy of rows [start, start+count) between a System array
+ * and device memory (to_system != 0: into the System), ordered on the
+ * System's stream */
+int cmag_system_copy_array_device(cmag_system* s, int array, float* device_data, int start, int count,
+                                  int to_system);
+
 /* ---- batches: many independent systems in one step ------------------------
  * No reference counterpart: the reference's process-global __constant__
  * parameters (fluid_kernel.cu:17, fluidSystem.cu:17-19) allow one system per

@@ -64,7 +64,16 @@ void oracle_calc_hash_f64(const cmag_params* P, const double* pos4, uint32_t* ha
     void oracle_configure_boundary##SFX(const cmag_params* P, R* pos, R wave_height,     \
                                         uint32_t n);                                     \
     void oracle_shift_right_boundary##SFX(const cmag_params* P, R* pos, uint32_t n);     \
-    void oracle_remove_right_boundary##SFX(const cmag_params* P, R* pos, uint32_t n);
+    void oracle_remove_right_boundary##SFX(const cmag_params* P, R* pos, uint32_t n);     \
+    void oracle_density_vjp##SFX(const cmag_params* P, R* gpos, const R* grho,           \
+                                 const R* sortedPos, const uint32_t* cellStart,          \
+                                 const uint32_t* cellEnd, uint32_t n);                   \
+    void oracle_force_vjp##SFX(const cmag_params* P, R* gpos, R* gvel, R* grho, R* gmu,  \
+                               const R* gacc, const R* measures, const R* sortedPos,     \
+                               const R* sortedVel, const uint32_t* cellStart,            \
+                               const uint32_t* cellEnd, uint32_t n);                     \
+    void oracle_integrate_vjp##SFX(const cmag_params* P, const R* pos, R* gpos, R* gvel,  \
+                                   R* gvelLF, R* gacc, R* ggrav, uint32_t n);
 
 CMAG_ORACLE_DECL(float, _f32)
 CMAG_ORACLE_DECL(double, _f64)

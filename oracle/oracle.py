@@ -281,3 +281,104 @@ def run_pipeline(P, pos, vel_lf, measures=None, dtype=np.float32):
     meas = density(P, sp, cs, ce, measures_in=measures, dtype=dtype)
     return dict(hash=h, sorted_hash=sh, index=perm, cell_start=cs, cell_end=ce, sorted_pos=sp,
                 sorted_vel=sv, measures=meas)
+
+
+# ---------------------------------------------------------------- adjoint (Poiseuille)
+def density_vjp(P, gpos_sorted, grho, sorted_pos, cell_start, cell_end, dtype=np.float64):
+    """gpos_sorted += d(rho . grho)/d(sorted positions) (walls skipped)"""
+    g = np.array(gpos_sorted, dtype=dtype, copy=True)
+    gr, sp = _f(grho, dtype), _f(sorted_pos, dtype)
+    cs, ce = _f(cell_start, np.uint32), _f(cell_end, np.uint32)
+    getattr(lib(), "oracle_density_vjp" + _sfx(dtype))(C.byref(P), _ptr(g), _ptr(gr), _ptr(sp), _ptr(cs),
+                                                      _ptr(ce), C.c_uint32(sp.shape[0]))
+    return g
+
+
+def force_vjp(P, gacc_sorted, measures, sorted_pos, sorted_vel, cell_start, cell_end, dtype=np.float64):
+    """adjoints of sorted positions, leap-frog velocities, densities and the
+    per-target share of d/d mu, from the sorted acceleration adjoint"""
+    ga, m = _f(gacc_sorted, dtype), _f(measures, dtype)
+    sp, sv = _f(sorted_pos, dtype), _f(sorted_vel, dtype)
+    cs, ce = _f(cell_start, np.uint32), _f(cell_end, np.uint32)
+    n = sp.shape[0]
+    gpos = np.zeros((n, 4), dtype); gvel = np.zeros((n, 4), dtype)
+    grho = np.zeros(n, dtype); gmu = np.zeros(n, dtype)
+    getattr(lib(), "oracle_force_vjp" + _sfx(dtype))(C.byref(P), _ptr(gpos), _ptr(gvel), _ptr(grho), _ptr(gmu),
+                                                    _ptr(ga), _ptr(m), _ptr(sp), _ptr(sv), _ptr(cs), _ptr(ce),
+                                                    C.c_uint32(n))
+    return gpos, gvel, grho, gmu
+
+
+def integrate_vjp(P, pos, gpos, gvel, gvel_lf, ggrav, dtype=np.float64):
+    """in place on (gpos, gvel, gvel_lf, ggrav); returns gacc"""
+    p = _f(pos, dtype)
+    n = p.shape[0]
+    gacc = np.zeros((n, 4), dtype)
+    for a in (gpos, gvel, gvel_lf, ggrav):
+        assert a.dtype == np.dtype(dtype) and a.flags.c_contiguous
+    getattr(lib(), "oracle_integrate_vjp" + _sfx(dtype))(C.byref(P), _ptr(p), _ptr(gpos), _ptr(gvel),
+                                                        _ptr(gvel_lf), _ptr(gacc), _ptr(ggrav), C.c_uint32(n))
+    return gacc
+
+
+def profile_loss(P, pos, vel, target, bins: int = 64):
+    """L = sum over non-empty y-bins of (mean x-velocity of the fluid in the
+    bin - target)^2; bins split [wall_bottom, wall_top] evenly.  Returns
+    (L, profile (nan where empty), dL/dvel (n,4))."""
+    pos = np.asarray(pos, np.float64); vel = np.asarray(vel, np.float64)
+    fl = pos[:, 3] == 0.0
+    lo, hi = float(P.wall_bottom), float(P.wall_top)
+    b = np.clip(np.floor((pos[fl, 1] - lo) / ((hi - lo) / bins)).astype(np.int64), 0, bins - 1)
+    cnt = np.bincount(b, minlength=bins).astype(np.float64)
+    s = np.bincount(b, weights=vel[fl, 0], minlength=bins)
+    nz = cnt > 0
+    prof = np.full(bins, np.nan)
+    prof[nz] = s[nz] / cnt[nz]
+    diff = np.where(nz, prof - np.asarray(target, np.float64), 0.0)
+    L = float(np.sum(diff[nz] ** 2))
+    g = np.zeros_like(vel)
+    g[np.flatnonzero(fl), 0] = 2.0 * diff[b] / cnt[b]
+    return L, prof, g
+
+
+def rollout(P, pos, vel, vel_lf, steps: int, dtype=np.float64, checkpoint_every: int = 0):
+    sim = Simulation(P, pos, vel, vel_lf, dtype=dtype)
+    cps = {}
+    for t in range(steps):
+        if checkpoint_every and t % checkpoint_every == 0:
+            cps[t] = (sim.pos.copy(), sim.vel.copy(), sim.velLF.copy())
+        sim.update()
+    return sim, cps
+
+
+def poiseuille_adjoint(P, pos, vel, vel_lf, steps: int, target, bins: int = 64, checkpoint_every: int = 20,
+                       dtype=np.float64) -> dict:
+    """Reverse-mode gradient of profile_loss after `steps` updates with
+    respect to mu (P.mu) and the body force P.gravity, by the hand adjoint:
+    checkpoints every `checkpoint_every` steps, each segment recomputed
+    forward and then walked backwards step by step."""
+    sim, cps = rollout(P, pos, vel, vel_lf, steps, dtype, checkpoint_every)
+    L, prof, gvel = profile_loss(P, sim.pos, sim.vel, target, bins)
+    n = sim.n
+    gvel = gvel.astype(dtype)
+    gpos = np.zeros((n, 4), dtype); glf = np.zeros((n, 4), dtype); ggrav = np.zeros((n, 4), dtype)
+    gmu = 0.0
+    for c in sorted(cps, reverse=True):
+        t1 = min(c + checkpoint_every, steps)
+        s = Simulation(P, *cps[c], dtype=dtype)
+        seg = []
+        for t in range(c, t1):
+            seg.append((s.pos.copy(), s.vel.copy(), s.velLF.copy()))
+            if t < t1 - 1:
+                s.update()
+        for spos, svel, slf in reversed(seg):
+            ref = run_pipeline(P, spos, slf, dtype=dtype)
+            gacc = integrate_vjp(P, spos, gpos, gvel, glf, ggrav, dtype)
+            idx = ref["index"].astype(np.int64)
+            gp_s, gl_s, grho, gmu_s = force_vjp(P, gacc[idx], ref["measures"], ref["sorted_pos"],
+                                                ref["sorted_vel"], ref["cell_start"], ref["cell_end"], dtype)
+            gp_s = density_vjp(P, gp_s, grho, ref["sorted_pos"], ref["cell_start"], ref["cell_end"], dtype)
+            gpos[idx] += gp_s
+            glf[idx] = gl_s
+            gmu += float(gmu_s.sum())
+    return dict(loss=L, profile=prof, dmu=gmu, dgrav=ggrav[:, :3].sum(axis=0), gpos=gpos, gvel=gvel, gvel_lf=glf)

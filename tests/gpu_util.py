@@ -156,3 +156,45 @@ def normwise(got: np.ndarray, want: np.ndarray) -> float:
     if scale == 0.0:
         return float(np.max(np.abs(got)))
     return float(np.max(np.abs(got.astype(np.float64) - want.astype(np.float64))) / scale)
+
+
+class AdjointStages:
+    """the adjoint stage launchers (cmag_*_vjp) over torch buffers"""
+
+    @staticmethod
+    def integrate_vjp(p, pos, gpos, gvel, gvlf):
+        n = pos.shape[0]
+        dp = to_dev(pos.astype(np.float32))
+        dgp, dgv, dgl = (to_dev(a.astype(np.float32)) for a in (gpos, gvel, gvlf))
+        dga = empty_f4(n)
+        dgg = torch.zeros((max(n, 1), 4), dtype=torch.float64, device="cuda")
+        L.call("cmag_integrate_vjp", P(p), ptr(dp), ptr(dgp), ptr(dgv), ptr(dgl), ptr(dga), ptr(dgg), n, None)
+        out = tuple(to_np(t)[:n] for t in (dgp, dgv, dgl, dga, dgg))
+        del dp
+        return out
+
+    @staticmethod
+    def force_vjp(p, gacc_sorted, measures, sorted_pos, sorted_vel, cs, ce):
+        n = sorted_pos.shape[0]
+        gp, gv = empty_f4(n), empty_f4(n)
+        gr = torch.zeros(max(n, 1), dtype=torch.float32, device="cuda")
+        gm = torch.zeros(max(n, 1), dtype=torch.float32, device="cuda")
+        dga, dm = to_dev(gacc_sorted.astype(np.float32)), to_dev(measures.astype(np.float32))
+        dsp, dsv = to_dev(sorted_pos.astype(np.float32)), to_dev(sorted_vel.astype(np.float32))
+        dcs, dce = to_dev(cs), to_dev(ce)
+        L.call("cmag_force_vjp", P(p), ptr(gp), ptr(gv), ptr(gr), ptr(gm), ptr(dga), ptr(dm), ptr(dsp), ptr(dsv),
+               ptr(dcs), ptr(dce), n, None)
+        out = to_np(gp)[:n], to_np(gv)[:n], to_np(gr)[:n], to_np(gm)[:n]
+        del dga, dm, dsp, dsv, dcs, dce
+        return out
+
+    @staticmethod
+    def density_vjp(p, gpos_sorted, grho, sorted_pos, cs, ce):
+        n = sorted_pos.shape[0]
+        gp = to_dev(gpos_sorted.astype(np.float32))
+        dgr, dsp = to_dev(grho.astype(np.float32)), to_dev(sorted_pos.astype(np.float32))
+        dcs, dce = to_dev(cs), to_dev(ce)
+        L.call("cmag_density_vjp", P(p), ptr(gp), ptr(dgr), ptr(dsp), ptr(dcs), ptr(dce), n, None)
+        out = to_np(gp)[:n]
+        del dgr, dsp, dcs, dce
+        return out

@@ -1,0 +1,129 @@
+"""The differentiable path on the GPU: the adjoint stage kernels against the
+oracle's adjoints, and the gradient of the binned velocity-profile loss of a
+200-step, 16,384-fluid-particle Poiseuille run with respect to viscosity and
+body force against the FP64 oracle's hand adjoint and central differences of
+its FP64 forward run (1e-3 relative)."""
+import numpy as np
+import pytest
+
+from cmag_amd import _lib as L
+from cmag_amd import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+
+def normwise(got, want):
+    return float(np.linalg.norm((got - want).ravel()) / max(np.linalg.norm(np.asarray(want).ravel()), 1e-300))
+
+
+def stage_case(name):
+    rng = np.random.default_rng(17)
+    if name == "channel":
+        sc = S.poiseuille_channel()
+    else:
+        sc = S.channel_poiseuille(3000, nu=2e-2, seed=11)
+    pos = sc.pos.copy()
+    vel = np.zeros_like(pos)
+    vel[:sc.num_fluid, :2] = rng.normal(scale=1e-2, size=(sc.num_fluid, 2))
+    return sc.params, pos, vel, sc.num_fluid, rng
+
+
+@pytest.mark.parametrize("name", ["channel", "batch_channel"])
+def test_force_and_density_vjp_match_oracle(device, oracle, name):
+    from tests.gpu_util import AdjointStages as A
+    p, pos, vel, nf, rng = stage_case(name)
+    ref = oracle.run_pipeline(p, pos, vel)
+    sp, sv, m, cs, ce = ref["sorted_pos"], ref["sorted_vel"], ref["measures"], ref["cell_start"], ref["cell_end"]
+    ga = np.zeros((pos.shape[0], 4), np.float32)
+    fluid = sp[:, 3] == 0
+    ga[fluid, :2] = rng.normal(size=(fluid.sum(), 2))
+    gp, gv, gr, gm = A.force_vjp(p, ga, m, sp, sv, cs, ce)
+    want = oracle.force_vjp(p, ga, m, sp, sv, cs, ce, dtype=np.float64)
+    for got, w in zip((gp[:, :3], gv[:, :3], gr, gm), (want[0][:, :3], want[1][:, :3], want[2], want[3])):
+        assert normwise(got, w) <= 1e-4, normwise(got, w)
+    gd = A.density_vjp(p, gp, gr, sp, cs, ce)
+    wd = oracle.density_vjp(p, want[0], want[2], sp, cs, ce, dtype=np.float64)
+    assert normwise(gd[:, :3], wd[:, :3]) <= 1e-4
+
+
+def test_integrate_vjp_matches_oracle(device, oracle):
+    from tests.gpu_util import AdjointStages as A
+    p, pos, vel, nf, rng = stage_case("channel")
+    n = pos.shape[0]
+    g = [rng.normal(size=(n, 4)).astype(np.float32) for _ in range(3)]
+    got = A.integrate_vjp(p, pos, *g)
+    w = [a.astype(np.float64).copy() for a in g]
+    ggrav = np.zeros((n, 4))
+    gacc = oracle.integrate_vjp(p, pos, w[0], w[1], w[2], ggrav, dtype=np.float64)
+    for a, b in zip(got, (w[0], w[1], w[2], gacc, ggrav)):
+        assert np.allclose(a, b, rtol=1e-6, atol=1e-6 * np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def adjoint_problem():
+    """the 200-step channel: target at (nu*, g*), gradient at (nu0, g0)"""
+    from oracle import oracle as O
+    d = S.adjoint_channel()
+    sc = d["scenario"]
+    simt, _ = O.rollout(d["params_star"], sc.pos, sc.vel, sc.vel, d["steps"])
+    _, target, _ = O.profile_loss(d["params0"], simt.pos, simt.vel, np.zeros(d["bins"]), d["bins"])
+    d["target"] = np.nan_to_num(target)
+    return d
+
+
+def gpu_gradient(d, checkpoint_every=20):
+    import torch
+
+    from cmag_amd.adjoint import ChannelRollout, profile_loss
+    sc, P = d["scenario"], d["params0"]
+    run = ChannelRollout(P, sc.pos, sc.vel, steps=d["steps"], checkpoint_every=checkpoint_every)
+    mu = torch.tensor(float(P.mu), dtype=torch.float64, requires_grad=True)
+    g = torch.tensor([float(P.gravity[a]) for a in range(3)], dtype=torch.float64, requires_grad=True)
+    pos, vel = run(mu, g)
+    loss = profile_loss(pos, vel, P, d["target"], d["bins"])
+    loss.backward()
+    return float(loss.detach()), float(mu.grad), float(g.grad[0])
+
+
+def test_profile_gradient_matches_fp64_adjoint_and_differences(device, oracle, adjoint_problem):
+    d = adjoint_problem
+    sc, P = d["scenario"], d["params0"]
+    loss, dmu, dg = gpu_gradient(d)
+    ref = oracle.poiseuille_adjoint(P, sc.pos, sc.vel, sc.vel, d["steps"], d["target"], bins=d["bins"],
+                                    checkpoint_every=d["checkpoint_every"], dtype=np.float64)
+    # the gradient with respect to nu is rho0 times the one with respect to mu
+    print(f"loss {loss:.6e} vs {ref['loss']:.6e}; dL/dnu {1000 * dmu:.6e} vs {1000 * ref['dmu']:.6e}; "
+          f"dL/dg {dg:.6e} vs {ref['dgrav'][0]:.6e}")
+    assert abs(loss - ref["loss"]) <= 1e-4 * ref["loss"]
+    assert abs(dmu - ref["dmu"]) <= 1e-3 * abs(ref["dmu"])
+    assert abs(dg - ref["dgrav"][0]) <= 1e-3 * abs(ref["dgrav"][0])
+
+    def fp64_loss(mu=None, g=None):
+        Q = P.copy()
+        if mu is not None:
+            Q.mu = mu
+        if g is not None:
+            Q.gravity[0] = g
+        s, _ = oracle.rollout(Q, sc.pos, sc.vel, sc.vel, d["steps"])
+        return oracle.profile_loss(Q, s.pos, s.vel, d["target"], d["bins"])[0]
+
+    eps = 1e-4
+    hi, lo = np.float32(float(P.mu) * (1 + eps)), np.float32(float(P.mu) * (1 - eps))
+    fd_mu = (fp64_loss(mu=hi) - fp64_loss(mu=lo)) / (float(hi) - float(lo))
+    g0 = float(P.gravity[0])
+    hi, lo = np.float32(g0 * (1 + eps)), np.float32(g0 * (1 - eps))
+    fd_g = (fp64_loss(g=hi) - fp64_loss(g=lo)) / (float(hi) - float(lo))
+    print(f"central differences: dL/dnu {1000 * fd_mu:.6e}, dL/dg {fd_g:.6e}")
+    assert abs(dmu - fd_mu) <= 1e-3 * abs(fd_mu)
+    assert abs(dg - fd_g) <= 1e-3 * abs(fd_g)
+
+
+def test_gradient_independent_of_checkpoint_spacing(device, adjoint_problem):
+    """recomputed segments replay the forward run exactly, so the gradient is
+    the same bit for bit whatever the checkpoint spacing"""
+    d = dict(adjoint_problem)
+    d["steps"] = 60
+    a = gpu_gradient(d, checkpoint_every=20)
+    b = gpu_gradient(d, checkpoint_every=7)
+    c = gpu_gradient(d, checkpoint_every=60)
+    assert a == b == c

@@ -708,3 +708,88 @@ def test_reference_poiseuille_report_matches_series(oracle):
         assert y.size == 58
         want = S.poiseuille_series(y, sim.elapsed, 1e-4, 1e-3, 1e-6)
         assert np.linalg.norm(u - want) / np.linalg.norm(want) < 0.01
+
+
+# ---------------------------------------------------------------- adjoint pins
+BINS = 16
+
+
+def _small_adjoint_case(O):
+    sc = S.poiseuille_channel(nx=16, ny=32, nu=2e-2, vmax=1e-2)
+    P = sc.params
+    rng = np.random.default_rng(0)
+    vel = np.zeros_like(sc.pos)
+    vel[:sc.num_fluid, :2] = 1e-3 * rng.normal(size=(sc.num_fluid, 2))
+    Pt = P.copy()
+    Pt.mu = np.float32(P.mu * 1.5)
+    Pt.gravity[0] = np.float32(P.gravity[0] * 0.7)
+    steps = 30
+    simt, _ = O.rollout(Pt, sc.pos, vel, vel, steps)
+    # 16 bins of 2 dx: lattice rows sit mid-bin, so no particle is near a bin
+    # edge, where the piecewise-constant binning would make differences jump
+    _, target, _ = O.profile_loss(P, simt.pos, simt.vel, np.zeros(BINS), BINS)
+    return P, sc, vel, steps, np.nan_to_num(target)
+
+
+def _fd(O, P, sc, vel, steps, target, which, eps=1e-4):
+    """central difference of the FP64 loss; the perturbed parameters are
+    float32 numbers (the parameter block's type), so the quotient uses their
+    exact difference"""
+    def loss(val):
+        Q = P.copy()
+        if which == "mu":
+            Q.mu = val
+        else:
+            Q.gravity[0] = val
+        s, _ = O.rollout(Q, sc.pos, vel, vel, steps)
+        return O.profile_loss(Q, s.pos, s.vel, target, BINS)[0]
+    x = float(P.mu) if which == "mu" else float(P.gravity[0])
+    hi, lo = np.float32(x * (1 + eps)), np.float32(x * (1 - eps))
+    return (loss(hi) - loss(lo)) / (float(hi) - float(lo))
+
+
+def test_adjoint_matches_central_differences(oracle):
+    """the FP64 hand adjoint (density, force with Morris walls, integrate;
+    checkpoints every 20 of 30 steps) against central differences of the FP64
+    forward run: a dropped term, a wrong sign or a missed neighbour side
+    moves the gradient far beyond the 1e-6 bound"""
+    P, sc, vel, steps, target = _small_adjoint_case(oracle)
+    res = oracle.poiseuille_adjoint(P, sc.pos, vel, vel, steps, target, bins=BINS, checkpoint_every=20)
+    fd_mu = _fd(oracle, P, sc, vel, steps, target, "mu")
+    fd_g = _fd(oracle, P, sc, vel, steps, target, "g")
+    assert abs(res["dmu"] - fd_mu) <= 1e-6 * abs(fd_mu)
+    assert abs(res["dgrav"][0] - fd_g) <= 1e-6 * abs(fd_g)
+
+
+def test_adjoint_initial_state_gradient(oracle):
+    """the adjoint of the initial leap-frog velocity (which only the force pass
+    reads) and of the initial positions against central differences along a
+    random direction"""
+    P, sc, vel, steps, target = _small_adjoint_case(oracle)
+    steps = 6
+    res = oracle.poiseuille_adjoint(P, sc.pos, vel, vel, steps, target, bins=BINS, checkpoint_every=4)
+    rng = np.random.default_rng(3)
+    fl = sc.pos[:, 3] == 0
+    for key, scale in (("gvel_lf", 1e-4), ("gpos", 1e-6)):
+        d = np.zeros((sc.pos.shape[0], 4))
+        d[fl, :2] = rng.normal(size=(fl.sum(), 2))
+        def loss(sign):
+            pos = sc.pos.astype(np.float64).copy()
+            lf = vel.astype(np.float64).copy()
+            if key == "gpos":
+                pos += sign * scale * d
+            else:
+                lf += sign * scale * d
+            s, _ = oracle.rollout(P, pos, vel.astype(np.float64), lf, steps)
+            return oracle.profile_loss(P, s.pos, s.vel, target, BINS)[0]
+        fd = (loss(1) - loss(-1)) / (2 * scale)
+        ad = float(np.sum(res[key] * d))
+        assert abs(ad - fd) <= 1e-4 * abs(fd) + 1e-18, (key, ad, fd)
+
+
+def test_adjoint_fp32_close_to_fp64(oracle):
+    P, sc, vel, steps, target = _small_adjoint_case(oracle)
+    a64 = oracle.poiseuille_adjoint(P, sc.pos, vel, vel, steps, target, bins=BINS, dtype=np.float64)
+    a32 = oracle.poiseuille_adjoint(P, sc.pos, vel, vel, steps, target, bins=BINS, dtype=np.float32)
+    assert abs(a32["dmu"] - a64["dmu"]) <= 1e-4 * abs(a64["dmu"])
+    assert abs(a32["dgrav"][0] - a64["dgrav"][0]) <= 1e-4 * abs(a64["dgrav"][0])
