@@ -14,6 +14,11 @@ resident in HBM before the timed region.
 Prints one JSON line: particle-steps/s (fluid particles), with
   roofline      the dominant kernel's algorithmic HBM bytes per launch over its
                 HIP-event-measured average duration, against 8 TB/s
+  batched       256 uneven 2-D channels stepped together (HIP graphs,
+                autotuned launch shape and cell size) against the same
+                instances as separate System objects stepped in a loop
+  adjoint       gradient of a 200-step 16k-particle Poiseuille profile loss
+                (forward + hand-written backward) against the plain forward run
   cpu_baseline  the oracle's FP32 update() on this host's cores, on the
                 1,024,000-fluid-particle dam break (a bounded sample of steps)
 """
@@ -182,6 +187,70 @@ def bench_batch(steps: int = 1000, plain_steps: int = 20, tune_cells: bool = Tru
     }
 
 
+def bench_adjoint(reps: int = 3) -> dict:
+    """The differentiable workload: gradient of the 64-bin x-velocity profile
+    loss of a 200-step, 16,384-fluid-particle Poiseuille run with respect to
+    viscosity and body force (checkpoints every 20 steps), timed forward +
+    backward against the plain 200-step forward run of the same system."""
+    import torch
+
+    from cmag_amd import scenarios as S
+    from cmag_amd.adjoint import ChannelRollout, profile, profile_loss
+    from cmag_amd.systems import ParamSystem
+
+    d = S.adjoint_channel()
+    sc, P0, Ps = d["scenario"], d["params0"], d["params_star"]
+    steps, bins = d["steps"], d["bins"]
+    n = sc.pos.shape[0]
+    # target profile from (nu*, g*) on the GPU's own forward run
+    tgt_run = ChannelRollout(Ps, sc.pos, sc.vel, steps=steps, checkpoint_every=d["checkpoint_every"])
+    with torch.no_grad():
+        mu_s = torch.tensor(float(Ps.mu), dtype=torch.float64)
+        g_s = torch.tensor([float(Ps.gravity[a]) for a in range(3)], dtype=torch.float64)
+        pos_t, vel_t = tgt_run(mu_s, g_s)
+    target = np.nan_to_num(profile(pos_t, vel_t, P0, bins))
+    run = ChannelRollout(P0, sc.pos, sc.vel, steps=steps, checkpoint_every=d["checkpoint_every"])
+
+    def grad_once():
+        mu = torch.tensor(float(P0.mu), dtype=torch.float64, requires_grad=True)
+        g = torch.tensor([float(P0.gravity[a]) for a in range(3)], dtype=torch.float64, requires_grad=True)
+        pos, vel = run(mu, g)
+        loss = profile_loss(pos, vel, P0, target, bins)
+        loss.backward()
+        torch.cuda.synchronize()
+        return float(loss.detach()), float(mu.grad), float(g.grad[0])
+
+    grad_once()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, dmu, dg = grad_once()
+    t_grad = (time.perf_counter() - t0) / reps
+
+    plain = ParamSystem.from_state(P0, sc.pos, sc.vel)
+    plain.run(steps, use_graph=True)
+    plain.synchronize()
+    plain = ParamSystem.from_state(P0, sc.pos, sc.vel)
+    plain.synchronize()
+    t0 = time.perf_counter()
+    plain.run(steps, use_graph=True)
+    plain.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+    return {
+        "workload": (f"2-D Poiseuille, {sc.num_fluid:,} fluid + {n - sc.num_fluid:,} wall particles, {steps} steps, "
+                     f"checkpoint every {d['checkpoint_every']}, loss over {bins} y-bins, target at "
+                     f"nu*={d['nu_star']}, gradient at nu0={d['nu0']}"),
+        "loss": loss,
+        "dL_dnu": 1000.0 * dmu,
+        "dL_dg": dg,
+        "forward_backward_ms": 1e3 * t_grad,
+        "plain_forward_ms": 1e3 * t_plain,
+        "ratio_vs_plain": t_grad / t_plain,
+        "particle_steps_per_s": n * steps / t_grad,
+        "plain_particle_steps_per_s": n * steps / t_plain,
+    }
+
+
 def cpu_baseline(target_seconds: float = 12.0) -> dict:
     from cmag_amd import scenarios as S
     from oracle import oracle as O
@@ -225,8 +294,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-channel comparison")
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--workload", choices=["dambreak", "batch"], default="dambreak",
-                    help="batch: only the batched-channel comparison (for profiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-run comparison")
+    ap.add_argument("--workload", choices=["dambreak", "batch", "adjoint"], default="dambreak",
+                    help="batch / adjoint: only that comparison (for profiling)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the timed steps, run 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
@@ -240,6 +310,9 @@ def main() -> int:
     if args.workload == "batch":
         res = bench_batch(steps=args.batch_steps)
         print(json.dumps({"batched": res}), flush=True)
+        return 0
+    if args.workload == "adjoint":
+        print(json.dumps({"adjoint": bench_adjoint()}), flush=True)
         return 0
 
     from cmag_amd import _lib as L
@@ -332,6 +405,8 @@ def main() -> int:
     }
     if not args.no_batch:
         result["batched"] = bench_batch(steps=args.batch_steps)
+    if not args.no_adjoint:
+        result["adjoint"] = bench_adjoint()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result), flush=True)
