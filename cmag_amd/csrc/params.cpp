@@ -48,6 +48,8 @@ int validate_params(const cmag_params* p) {
     if (p->system < CMAG_SYS_DAMBREAK || p->system > CMAG_SYS_PERISTALSIS)
         return fail(CMAG_E_INVALID, "unknown system");
     if (p->dim != 2 && p->dim != 3) return fail(CMAG_E_INVALID, "dim must be 2 or 3");
+    if (p->system == CMAG_SYS_DAMBREAK && p->periodic_mask)
+        return fail(CMAG_E_INVALID, "the DamBreak system has no periodic images");
     if (p->eos == CMAG_EOS_TAIT &&
         (p->gamma < 1.0f || p->gamma > 15.0f || p->gamma != std::floor(p->gamma)))
         return fail(CMAG_E_INVALID, "Tait gamma must be an integer in [1,15]");

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void count_kernel(const cmag_params P, uint32_
     walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int) {
         const float4 pj = sortedPos[j];
         const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
-        c += (j != i && rx * rx + ry * ry + rz * rz < r2max) ? 1u : 0u;
+        c += (j != i && norm2(rx, ry, rz) < r2max) ? 1u : 0u;
     });
     counts[i] = c;
 }

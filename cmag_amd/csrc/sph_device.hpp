@@ -19,10 +19,22 @@
 
 #include "common.hpp"
 
+// Every fused multiply-add below is written out (fmaf) and no other is formed:
+// the same sum then rounds the same way in every kernel that inlines it
+// (walker or neighbour list, single system or batch), which keeps those
+// paths bit-identical.
+#pragma clang fp contract(off)
+
 namespace cmag {
+
+// |r|^2 as (x*x) then two fused steps
+__device__ __forceinline__ float norm2(float rx, float ry, float rz) {
+    return fmaf(rz, rz, fmaf(ry, ry, rx * rx));
+}
 
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own
@@ -103,14 +115,16 @@ struct Run {
 // first and then walked as one loop, so every lane of a wave spends the row in
 // the same loop whatever its x position -- lanes whose row splits at a
 // different x would otherwise serialise the wave.
-template <int DIM, int HASH, typename Cells, typename C>
+// PERIODIC = false compiles the image shifts out (DamBreak has none).
+template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
 __device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4& pi, const Cell& gp,
                                              const Cells& cells, C&& cand) {
     const int cc = P.cellcount;
     const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
     const uint32_t gxm = P.grid_size[0] - 1u, gym = P.grid_size[1] - 1u, gzm = P.grid_size[2] - 1u;
     const uint32_t Mx = P.morton_mask[0], My = P.morton_mask[1], Mz = P.morton_mask[2];
-    const bool perx = P.periodic_mask & 1, pery = P.periodic_mask & 2, perz = P.periodic_mask & 4;
+    const bool perx = PERIODIC && (P.periodic_mask & 1), pery = PERIODIC && (P.periodic_mask & 2),
+               perz = PERIODIC && (P.periodic_mask & 4);
     uint32_t kx0 = 0, ky0 = 0, kz0 = 0;
     if (HASH == CMAG_HASH_MORTON) {
         const uint32_t own = grid_hash(P, gp.x, gp.y, gp.z);
@@ -193,6 +207,23 @@ __device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4&
     }
 }
 
+// row j of a float4 array through a 32-bit byte offset (j < 2^28), which
+// lets the load take the array base from scalar registers
+__device__ __forceinline__ float4 row4(const float4* __restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (j << 4));
+}
+
+constexpr int NB_CAP = 16;
+constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
+
+struct NbEntry {
+    __device__ static __forceinline__ uint32_t pack(uint32_t j, int code) {
+        return j | ((uint32_t)code << NB_SHIFT_BITS);
+    }
+    __device__ static __forceinline__ uint32_t index(uint32_t e) { return e & ((1u << NB_SHIFT_BITS) - 1u); }
+    __device__ static __forceinline__ int code(uint32_t e) { return (int)(e >> NB_SHIFT_BITS); }
+};
+
 // ---------------------------------------------------------------------------
 template <int SYS, int DIM, int HASH, typename Cells>
 __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, float4* __restrict__ measures,
@@ -204,20 +235,17 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
         return;
     }
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
+    const float inv_h = 1.0f / P.smoothing_radius;
     float sum = 0.0f;
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int) {
-        const float4 pj = sortedPos[j];
-        const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < r2max) {
-            const float q = fsqrt(r2) * inv_h;
-            const float t = 1.0f - 0.5f * q;
-            const float t2 = t * t;
-            sum += t2 * t2 * (2.0f * q + 1.0f);
-        }
+    walk_stencil<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, [&](uint32_t j, float px, float py,
+                                                                                  float pz, int) {
+        const float4 pj = row4(sortedPos, j);
+        const float r2 = norm2(px - pj.x, py - pj.y, pz - pj.z);
+        // (1 - q/2) clamped at 0 contributes nothing beyond 2h: no branch
+        const float q = fsqrt(r2) * inv_h;
+        const float t = fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f);
+        const float t2 = t * t;
+        sum = fmaf(t2 * t2, fmaf(2.0f, q, 1.0f), sum);
     });
     const float dens = sum * P.kernel_coeff * P.particle_mass;
     if (SYS == CMAG_SYS_PERISTALSIS)
@@ -241,28 +269,11 @@ __device__ __forceinline__ float3 vel_diff_poiseuille(const cmag_params& P, cons
     return make_float3(vi.x - vj.x, vi.y - vj.y, vi.z - vj.z);
 }
 
-// ---- force passes: two phases per target ------------------------------------
-// Phase 1 walks the stencil and only tests candidates (r^2 < 4h^2), appending
-// the in-range ones to a per-lane list in LDS; phase 2 evaluates the pair
-// terms of the listed neighbours.  Without the split, the ~50-instruction pair
-// term would run for every candidate (some lane of the wave is almost always
-// in range), i.e. ~475 times per particle instead of ~74.  The list holds
-// CAP entries per lane (NB_CAP unless a caller picks another depth); when a
-// lane fills it, every active lane flushes.
-constexpr int NB_CAP = 16;
-constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
-
-struct NbEntry {
-    __device__ static __forceinline__ uint32_t pack(uint32_t j, int code) {
-        return j | ((uint32_t)code << NB_SHIFT_BITS);
-    }
-    __device__ static __forceinline__ uint32_t index(uint32_t e) { return e & ((1u << NB_SHIFT_BITS) - 1u); }
-    __device__ static __forceinline__ int code(uint32_t e) { return (int)(e >> NB_SHIFT_BITS); }
-};
-
 // target position minus the periodic image offset named by code, computed with
 // the same float operations as walk_stencil
+template <bool PERIODIC = true>
 __device__ __forceinline__ float3 image_target(const cmag_params& P, const float4& pi, int code) {
+    if (!PERIODIC) return make_float3(pi.x, pi.y, pi.z);
     const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
     float3 q;
     q.x = (P.periodic_mask & 1) ? pi.x - (float)sx * P.world_size[0] : pi.x;
@@ -271,6 +282,81 @@ __device__ __forceinline__ float3 image_target(const cmag_params& P, const float
     return q;
 }
 
+// ---- fused force (DamBreak, Poiseuille) -------------------------------------------
+// Per-target state of the force sum.  Pair terms accumulate in f in walk
+// order; DamBreak's wall terms go to w as they are met, and f + w is formed
+// once at the end, so the result does not depend on how the neighbours were
+// staged (list depth, which particles share the wave, lists from the density
+// pass).
+template <int SYS>
+struct ForceSum {
+    const cmag_params& P;
+    float4 pi, vi;
+    float dens, inv_dens, pterm_i, inv_h, m, gc, av_k, av_eps, visc_k;
+    float fx = 0.f, fy = 0.f, fz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f;
+
+    __device__ __forceinline__ ForceSum(const cmag_params& P_, const float4& pi_, const float4& vi_, const float4& mi)
+        : P(P_), pi(pi_), vi(vi_) {
+        dens = mi.x;
+        inv_dens = 1.0f / dens;
+        pterm_i = mi.y / (dens * dens);
+        const float h = P.smoothing_radius;
+        inv_h = 1.0f / h;
+        m = P.particle_mass;
+        gc = P.grad_coeff;
+        av_k = 2.0f * P.alpha * h * P.soundspeed;  // nu * (rho_i + rho_j)
+        av_eps = 0.001f * h * h;
+        visc_k = m * (P.mu + P.mu);
+    }
+
+    // Lennard-Jones wall term (DamBreak), fluid_kernel.cu:199-208
+    __device__ __forceinline__ void wall(float rx, float ry, float rz, float r2) {
+        const float inv_d = frsq(r2);
+        const float ad = P.lj_a * inv_d;
+        const float ad2 = ad * ad;
+        const float ad6 = ad2 * ad2 * ad2;
+        const float sc = P.lj_D * (ad6 * ad6 - ad6) * (inv_d * inv_d);
+        wx = fmaf(sc, rx, wx); wy = fmaf(sc, ry, wy); wz = fmaf(sc, rz, wz);
+    }
+
+    // pressure + artificial viscosity (DamBreak) or + Morris viscosity
+    // (Poiseuille) of an in-range neighbour; one reciprocal square root and
+    // one reciprocal per pair (two with artificial viscosity)
+    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj, const float4& vj) {
+        const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
+        const float r2 = norm2(rx, ry, rz);
+        const float inv_d = frsq(r2);
+        const float dist = r2 * inv_d;
+        const float qq = dist * inv_h;
+        const float t = fmaf(-0.5f, qq, 1.0f);
+        const float t2 = t * t;
+        const float temp = gc * fmaf(t2, t2, -(t2 * t) * fmaf(2.0f, qq, 1.0f));
+        const float inv_dj = frcp(mj.x);
+        float s = fmaf(mj.y * inv_dj, inv_dj, pterm_i);
+        if (SYS == CMAG_SYS_DAMBREAK) {
+            const float vr = fmaf(vi.z - vj.z, rz, fmaf(vi.y - vj.y, ry, (vi.x - vj.x) * rx));
+            if (vr < 0.0f) s += -av_k * vr * frcp((dens + mj.x) * (r2 + av_eps));
+            const float kk = -m * s * inv_d * temp;
+            fx = fmaf(kk, rx, fx); fy = fmaf(kk, ry, fy); fz = fmaf(kk, rz, fz);
+        } else {
+            const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
+            const float kk = -m * s * inv_d * temp;
+            const float kv = visc_k * inv_dens * inv_dj * inv_d * temp;
+            fx += fmaf(kk, rx, kv * V.x);
+            fy += fmaf(kk, ry, kv * V.y);
+            fz += fmaf(kk, rz, kv * V.z);
+        }
+    }
+};
+
+// ---- force passes: two phases per target ------------------------------------
+// Phase 1 walks the stencil and only tests candidates (r^2 < 4h^2), appending
+// the in-range ones to a per-lane list in LDS; phase 2 evaluates the pair
+// terms of the listed neighbours.  Without the split, the ~50-instruction pair
+// term would run for every candidate (some lane of the wave is almost always
+// in range), i.e. ~475 times per particle instead of ~74.  The list holds
+// CAP entries per lane (NB_CAP unless a caller picks another depth); when a
+// lane fills it, every active lane flushes.
 // nbl: this wave's neighbour lists in LDS, [CAP][64]
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
@@ -282,25 +368,9 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     const int lane = threadIdx.x & 63;
     const float4 pi = sortedPos[i];
     if (SYS == CMAG_SYS_DAMBREAK && pi.w != CMAG_DB_FLUID) return;
-    const float4 vi = sortedVel[i];
-    const float4 mi = measures[i];
-    const float dens = mi.x;
-    const float pterm_i = mi.y / (dens * dens);
+    ForceSum<SYS> F(P, pi, sortedVel[i], measures[i]);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
-    const float m = P.particle_mass;
-    const float gc = P.grad_coeff;
-    const float av_k = 2.0f * P.alpha * h * P.soundspeed;  // nu * (rho_i + rho_j)
-    const float av_eps = 0.001f * h * h;
-    const float visc_k = m * (P.mu + P.mu);
-    // pair terms accumulate in f in walk order whenever the list is flushed;
-    // wall terms (DamBreak) go to w as they are met, and f + w is formed once
-    // at the end, so the result does not depend on when flushes happen (list
-    // depth, which particles share the wave)
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
     int count = 0;
 
     auto flush = [&]() {
@@ -308,50 +378,21 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
             if (k >= count) break;
             const uint32_t e = nbl[k][lane];
             const uint32_t j = NbEntry::index(e);
-            const float3 q = image_target(P, pi, NbEntry::code(e));
-            const float4 pj = sortedPos[j];
-            const float4 mj = measures[j];
-            const float4 vj = sortedVel[j];
-            const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float dist = fsqrt(r2);
-            const float inv_d = frcp(dist);
-            const float qq = dist * inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float t2 = t * t;
-            const float temp = gc * (t2 * t2 - t2 * t * (2.0f * qq + 1.0f));
-            const float inv_dj = frcp(mj.x);
-            float s = pterm_i + mj.y * inv_dj * inv_dj;
-            if (SYS == CMAG_SYS_DAMBREAK) {
-                const float vr = (vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz;
-                if (vr < 0.0f) s += -av_k * frcp(dens + mj.x) * vr * frcp(r2 + av_eps);
-                const float kk = -m * s * inv_d * temp;
-                fx += kk * rx; fy += kk * ry; fz += kk * rz;
-            } else {
-                const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
-                const float kk = -m * s * inv_d * temp;
-                const float kv = visc_k * frcp(dens * mj.x) * inv_d * temp;
-                fx += kk * rx + kv * V.x;
-                fy += kk * ry + kv * V.y;
-                fz += kk * rz + kv * V.z;
-            }
+            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e)), row4(sortedPos, j),
+                   row4(measures, j), row4(sortedVel, j));
         }
         count = 0;
     };
 
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float px, float py, float pz, int code) {
-        const float4 pj = sortedPos[j];
+    walk_stencil<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, [&](uint32_t j, float px, float py,
+                                                                                  float pz, int code) {
+        const float4 pj = row4(sortedPos, j);
         const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = norm2(rx, ry, rz);
         if (j != i) {
             if (SYS == CMAG_SYS_DAMBREAK && pj.w != CMAG_DB_FLUID) {
                 // Lennard-Jones wall term: every wall particle of the stencil acts
-                const float dist = fsqrt(r2);
-                const float ad = P.lj_a * frcp(dist);
-                const float ad2 = ad * ad;
-                const float ad6 = ad2 * ad2 * ad2;
-                const float sc = P.lj_D * (ad6 * ad6 - ad6) * frcp(dist * dist);
-                wx += sc * rx; wy += sc * ry; wz += sc * rz;
+                F.wall(rx, ry, rz, r2);
             } else if (r2 < r2max) {
                 nbl[count][lane] = NbEntry::pack(j, code);
                 ++count;
@@ -361,11 +402,51 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     });
     flush();
     const uint32_t k = index[i];
-    if (k < n) acc[k] = make_float4(fx + wx, fy + wy, fz + wz, 0.0f);
+    if (k < n) acc[k] = make_float4(F.fx + F.wx, F.fy + F.wy, F.fz + F.wz, 0.0f);
 }
 
-// Peristalsis: pressure and viscous passes; the fused variant produces both
-// from one neighbour walk (same per-pair arithmetic as the split kernels)
+// ---- Peristalsis: pressure and viscous passes -------------------------------------
+// The fused variant produces both from one neighbour walk (same per-pair
+// arithmetic as the split kernels)
+template <bool WANT_VISC, bool WANT_PRES>
+struct PeriSum {
+    const cmag_params& P;
+    float4 pi, vi;
+    float dens, inv_dens, pterm_i, inv_h, m, gc, visc_k;
+    float vx = 0.f, vy = 0.f, vz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+    __device__ __forceinline__ PeriSum(const cmag_params& P_, const float4& pi_, const float4& vi_, const float4& mi)
+        : P(P_), pi(pi_), vi(vi_) {
+        dens = mi.x;
+        inv_dens = 1.0f / dens;
+        pterm_i = mi.y / (dens * dens);
+        inv_h = 1.0f / P.smoothing_radius;
+        m = P.particle_mass;
+        gc = P.grad_coeff;
+        visc_k = m * (P.mu + P.mu);
+    }
+    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj,
+                                         const float4* __restrict__ sortedVel, uint32_t j) {
+        const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
+        const float r2 = norm2(rx, ry, rz);
+        const float inv_d = frsq(r2);
+        const float dist = r2 * inv_d;
+        const float qq = dist * inv_h;
+        const float t = fmaf(-0.5f, qq, 1.0f);
+        const float t2 = t * t;
+        const float temp = gc * fmaf(t2, t2, -(t2 * t) * fmaf(2.0f, qq, 1.0f));
+        const float inv_dj = frcp(mj.x);
+        if (WANT_PRES) {
+            const float kk = -m * temp * fmaf(mj.y * inv_dj, inv_dj, pterm_i) * inv_d;
+            px = fmaf(kk, rx, px); py = fmaf(kk, ry, py); pz = fmaf(kk, rz, pz);
+        }
+        if (WANT_VISC) {
+            const float4 vj = row4(sortedVel, j);
+            const float kv = visc_k * temp * inv_dens * inv_dj * inv_d;
+            vx = fmaf(kv, vi.x - vj.x, vx); vy = fmaf(kv, vi.y - vj.y, vy); vz = fmaf(kv, vi.z - vj.z, vz);
+        }
+    }
+};
+
 template <int DIM, int HASH, bool WANT_VISC, bool WANT_PRES, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void peristalsis_force_body(
     const cmag_params& P, uint32_t i, float4* __restrict__ viscF, float4* __restrict__ presF,
@@ -374,19 +455,10 @@ __device__ __forceinline__ void peristalsis_force_body(
     uint32_t (*nbl)[64]) {
     const int lane = threadIdx.x & 63;
     const float4 pi = sortedPos[i];
-    const float4 mi = measures[i];
-    const float dens = mi.x;
-    const float pterm_i = mi.y / (dens * dens);
-    float4 vi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (WANT_VISC) vi = sortedVel[i];
+    PeriSum<WANT_VISC, WANT_PRES> F(P, pi, WANT_VISC ? sortedVel[i] : make_float4(0.f, 0.f, 0.f, 0.f),
+                                    measures[i]);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float h = P.smoothing_radius;
-    const float inv_h = 1.0f / h;
-    const float r2max = 4.0f * h * h;
-    const float m = P.particle_mass;
-    const float gc = P.grad_coeff;
-    const float visc_k = m * (P.mu + P.mu);
-    float vx = 0.f, vy = 0.f, vz = 0.f, px = 0.f, py = 0.f, pz = 0.f;
+    const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
     int count = 0;
 
     auto flush = [&]() {
@@ -394,35 +466,15 @@ __device__ __forceinline__ void peristalsis_force_body(
             if (k >= count) break;
             const uint32_t e = nbl[k][lane];
             const uint32_t j = NbEntry::index(e);
-            const float3 q = image_target(P, pi, NbEntry::code(e));
-            const float4 pj = sortedPos[j];
-            const float4 mj = measures[j];
-            const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
-            const float r2 = rx * rx + ry * ry + rz * rz;
-            const float dist = fsqrt(r2);
-            const float inv_d = frcp(dist);
-            const float qq = dist * inv_h;
-            const float t = 1.0f - 0.5f * qq;
-            const float t2 = t * t;
-            const float temp = gc * (t2 * t2 - t2 * t * (2.0f * qq + 1.0f));
-            if (WANT_PRES) {
-                const float inv_dj = frcp(mj.x);
-                const float kk = -m * temp * (pterm_i + mj.y * inv_dj * inv_dj) * inv_d;
-                px += kk * rx; py += kk * ry; pz += kk * rz;
-            }
-            if (WANT_VISC) {
-                const float4 vj = sortedVel[j];
-                const float kv = visc_k * temp * frcp(dens * mj.x * dist);
-                vx += kv * (vi.x - vj.x); vy += kv * (vi.y - vj.y); vz += kv * (vi.z - vj.z);
-            }
+            F.pair(image_target(P, pi, NbEntry::code(e)), row4(sortedPos, j), row4(measures, j), sortedVel, j);
         }
         count = 0;
     };
 
     walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float qx, float qy, float qz, int code) {
-        const float4 pj = sortedPos[j];
+        const float4 pj = row4(sortedPos, j);
         const float rx = qx - pj.x, ry = qy - pj.y, rz = qz - pj.z;
-        const float r2 = rx * rx + ry * ry + rz * rz;
+        const float r2 = norm2(rx, ry, rz);
         if (j != i && r2 < r2max) {
             nbl[count][lane] = NbEntry::pack(j, code);
             ++count;
@@ -432,8 +484,8 @@ __device__ __forceinline__ void peristalsis_force_body(
     flush();
     const uint32_t k = index[i];
     if (k >= n) return;
-    if (WANT_VISC) viscF[k] = make_float4(vx, vy, vz, 0.0f);
-    if (WANT_PRES) presF[k] = make_float4(px, py, pz, 0.0f);
+    if (WANT_VISC) viscF[k] = make_float4(F.vx, F.vy, F.vz, 0.0f);
+    if (WANT_PRES) presF[k] = make_float4(F.px, F.py, F.pz, 0.0f);
 }
 
 }  // namespace cmag

@@ -128,6 +128,7 @@ class SystemBase:
     def synchronize(self) -> None:
         call("cmag_system_synchronize", self._h)
 
+
     # device-side state access (stream-ordered on the system's stream)
     def copy_from_device(self, array: int, device_ptr: int, count: int | None = None) -> None:
         """rows [0, count) of `array` <- device memory at device_ptr"""
