@@ -65,6 +65,14 @@ void oracle_calc_hash_f64(const cmag_params* P, const double* pos4, uint32_t* ha
                                         uint32_t n);                                     \
     void oracle_shift_right_boundary##SFX(const cmag_params* P, R* pos, uint32_t n);     \
     void oracle_remove_right_boundary##SFX(const cmag_params* P, R* pos, uint32_t n);     \
+    void oracle_density_subset##SFX(const cmag_params* P, R* measures, const R* sortedPos, \
+                                    const uint32_t* cellStart, const uint32_t* cellEnd,  \
+                                    const uint32_t* sel, uint32_t m);                    \
+    void oracle_force_subset##SFX(const cmag_params* P, R* acc, const R* measures,       \
+                                  const R* sortedPos, const R* sortedVel,                \
+                                  const uint32_t* index, const uint32_t* cellStart,      \
+                                  const uint32_t* cellEnd, const uint32_t* sel,          \
+                                  uint32_t m);                                           \
     void oracle_density_vjp##SFX(const cmag_params* P, R* gpos, const R* grho,           \
                                  const R* sortedPos, const uint32_t* cellStart,          \
                                  const uint32_t* cellEnd, uint32_t n);                   \

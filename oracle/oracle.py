@@ -382,3 +382,27 @@ def poiseuille_adjoint(P, pos, vel, vel_lf, steps: int, target, bins: int = 64, 
             glf[idx] = gl_s
             gmu += float(gmu_s.sum())
     return dict(loss=L, profile=prof, dmu=gmu, dgrav=ggrav[:, :3].sum(axis=0), gpos=gpos, gvel=gvel, gvel_lf=glf)
+
+
+def density_subset(P, sorted_pos, cell_start, cell_end, sel, measures_in=None, dtype=np.float32):
+    """density of the sorted particles sel only (rows of the others unchanged)"""
+    sp = _f(sorted_pos, dtype)
+    cs, ce, sl = _f(cell_start, np.uint32), _f(cell_end, np.uint32), _f(sel, np.uint32)
+    n = sp.shape[0]
+    meas = np.zeros((n, 4), dtype) if measures_in is None else np.array(measures_in, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_density_subset" + _sfx(dtype))(C.byref(P), _ptr(meas), _ptr(sp), _ptr(cs), _ptr(ce),
+                                                         _ptr(sl), C.c_uint32(sl.size))
+    return meas
+
+
+def force_subset(P, measures, sorted_pos, sorted_vel, index, cell_start, cell_end, sel, acc_in=None,
+                 dtype=np.float32):
+    """force of the sorted particles sel only, scattered to acc[index[sel]]"""
+    sp, m, sv = _f(sorted_pos, dtype), _f(measures, dtype), _f(sorted_vel, dtype)
+    idx, cs, ce = _f(index, np.uint32), _f(cell_start, np.uint32), _f(cell_end, np.uint32)
+    sl = _f(sel, np.uint32)
+    n = sp.shape[0]
+    acc = np.zeros((n, 4), dtype) if acc_in is None else np.array(acc_in, dtype=dtype, copy=True)
+    getattr(lib(), "oracle_force_subset" + _sfx(dtype))(C.byref(P), _ptr(acc), _ptr(m), _ptr(sp), _ptr(sv),
+                                                       _ptr(idx), _ptr(cs), _ptr(ce), _ptr(sl), C.c_uint32(sl.size))
+    return acc

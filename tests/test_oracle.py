@@ -793,3 +793,22 @@ def test_adjoint_fp32_close_to_fp64(oracle):
     a32 = oracle.poiseuille_adjoint(P, sc.pos, vel, vel, steps, target, bins=BINS, dtype=np.float32)
     assert abs(a32["dmu"] - a64["dmu"]) <= 1e-4 * abs(a64["dmu"])
     assert abs(a32["dgrav"][0] - a64["dgrav"][0]) <= 1e-4 * abs(a64["dgrav"][0])
+
+
+def test_subset_density_and_force_equal_full_passes(oracle):
+    """the per-particle entry points the large-state checks use give exactly
+    the full passes' rows"""
+    sc = S.dambreak3d(dx=0.04)
+    rng = np.random.default_rng(2)
+    vel = np.zeros_like(sc.pos)
+    vel[:sc.num_fluid, :3] = rng.normal(scale=0.1, size=(sc.num_fluid, 3)).astype(np.float32)
+    ref = oracle.run_pipeline(sc.params, sc.pos, vel)
+    sel = np.sort(rng.choice(sc.pos.shape[0], 300, replace=False)).astype(np.uint32)
+    d = oracle.density_subset(sc.params, ref["sorted_pos"], ref["cell_start"], ref["cell_end"], sel)
+    assert np.array_equal(d[sel, :2], ref["measures"][sel, :2])
+    full = oracle.force(sc.params, ref["measures"], ref["sorted_pos"], ref["sorted_vel"], ref["index"],
+                        ref["cell_start"], ref["cell_end"])
+    part = oracle.force_subset(sc.params, ref["measures"], ref["sorted_pos"], ref["sorted_vel"], ref["index"],
+                               ref["cell_start"], ref["cell_end"], sel)
+    rows = ref["index"][sel]
+    assert np.array_equal(part[rows], full[rows])
