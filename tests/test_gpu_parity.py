@@ -142,6 +142,8 @@ SORT_CASES = [
     (0, 8, "random"), (1, 8, "random"), (2, 1, "random"), (100, 14, "random"), (4095, 20, "random"),
     (4096, 20, "dups"), (4097, 24, "sorted"), (65536 + 3, 24, "reversed"), (300_000, 20, "dups"),
     (1_000_003, 24, "random"), (1 << 20, 32, "random"), (50_000, 16, "allsame"), (123_457, 9, "random"),
+    # a few tiles of 4096 keys, full 32-bit keys, partial last tiles
+    (20_000, 9, "dups"), (32_768, 32, "random"), (32_768, 17, "reversed"), (32_769, 24, "random"),
 ]
 
 
