@@ -164,3 +164,33 @@ def test_right_boundary_controls_on_device(device, oracle):
     s.removeRightBoundary()
     assert s.getElapsedTime() == 0.0
     assert s.params.right_boundary == np.float32(0xFFFFFFFF)
+
+
+def test_channel_2000_steps_matches_series_and_fp64_oracle(device, oracle):
+    """The CPU-runnable channel (32x64 fluid, Tait, Morris walls, h = 1.3 dx)
+    run for 2000 CFL steps on the GPU (HIP graph): its 64-bin x-velocity profile
+    is within 1% L2 of the analytic start-up series at the same time T (taken at
+    K*nu, K = the lattice's Laplacian moment) and within 1e-3 of the FP64
+    oracle's profile after the same 2000 steps."""
+    sc = S.poiseuille_channel()
+    steps = 2000
+    s = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    s.run(steps, use_graph=True)
+    pos, vel = s.getArray(L.ARRAY_POSITION), s.getArray(L.ARRAY_VELOCITY)
+
+    def profile(p, v):
+        fl = p[:, 3] == 0
+        y = p[fl, 1].astype(np.float64) - sc.meta["y0"]
+        b = np.clip((y / sc.meta["width"] * 64).astype(int), 0, 63)
+        return np.bincount(b, v[fl, 0], 64) / np.maximum(np.bincount(b, None, 64), 1)
+
+    got = profile(pos, vel)
+    T = steps * float(np.float32(sc.params.delta_time))
+    yc = (np.arange(64) + 0.5) / 64 * sc.meta["width"]
+    k = S.lattice_laplacian_moment(sc.meta["h"], sc.meta["dx"], 2)
+    want = S.poiseuille_series(yc, T, sc.meta["g"], sc.meta["width"], sc.meta["nu"] * k)
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 0.01
+    sim = oracle.Simulation(sc.params, sc.pos, sc.vel, sc.vel, dtype=np.float64)
+    sim.update(steps)
+    ref = profile(sim.pos, sim.vel)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-3
