@@ -2,13 +2,15 @@
 
 * the 1M-fluid-particle 3-D dam break (1,024,000 fluid + 571,308 wall
   particles): one update() and four more against the FP32 oracle, as at the
-  small sizes;
+  small sizes; then 500 steps with hash, sort and cell ranges checked bit for
+  bit and density and force against the oracle's loop bodies every 50 steps,
+  on the GPU's own state;
 * the 16M-fluid-particle dam break the benchmark times (16,484,816 fluid +
   3,586,416 wall particles, brick Z-order): after one update(), the sort is a
   stable permutation of the hashes of the initial positions, the cell ranges and
   the gather equal the oracle's on those keys bit for bit, and density and force
   of 20,000 sampled particles equal the oracle's loop bodies run on the GPU's
-  own sorted state."""
+  own sorted state; the same again after 200 steps."""
 import numpy as np
 import pytest
 
@@ -47,16 +49,15 @@ def test_dambreak_1m_matches_oracle(device, oracle):
     assert np.abs(pos[:, :3] - sim.pos[:, :3]).max() <= 1e-6 * np.abs(sim.pos[:, :3]).max()
 
 
-def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
-    sc = S.dambreak3d(dx=0.4 / 202, hash_order=L.HASH_MORTON, morton_xbits=4)
-    p, pos0 = sc.params, sc.pos
+def check_step_on_own_state(s, oracle, p, pos0, sample, rng):
+    """one update() from positions pos0 (the System's current state): the sort
+    is the stable permutation of pos0's hashes, cell ranges and gather equal the
+    oracle's on those keys, density and force of `sample` sorted particles (all
+    when None) equal the oracle's loop bodies on the GPU's sorted state"""
     n = pos0.shape[0]
-    print(f"16M dam break: {sc.num_fluid} fluid + {sc.meta['num_wall']} wall particles", flush=True)
-    s = ParamSystem.from_state(p, pos0, sc.vel)
     s.update()
     h = s.getUintArray(L.ARRAY_HASH)
     idx = s.getUintArray(L.ARRAY_INDEX)
-    # the keys of the initial positions, sorted stably
     want_h, _ = oracle.calc_hash(p, pos0)
     assert np.array_equal(np.sort(idx), np.arange(n, dtype=np.uint32))
     assert np.array_equal(h, want_h[idx])
@@ -64,8 +65,7 @@ def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
     same = h[1:] == h[:-1]
     assert np.all(idx[1:][same] > idx[:-1][same])
     cs, ce = oracle.find_cell_start(h, S.num_cells(p))
-    got_cs = s.getUintArray(L.ARRAY_CELLSTART)
-    assert np.array_equal(got_cs, cs)
+    assert np.array_equal(s.getUintArray(L.ARRAY_CELLSTART), cs)
     occ = cs != L.CELL_EMPTY
     assert np.array_equal(s.getUintArray(L.ARRAY_CELLEND)[occ], ce[occ])
     sp = s.getArray(L.ARRAY_SORTEDPOS)
@@ -73,8 +73,8 @@ def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
     sv = s.getArray(L.ARRAY_SORTEDVEL)
     meas = s.getArray(L.ARRAY_MEASURES)
     acc = s.getArray(L.ARRAY_ACCELERATION)
-    rng = np.random.default_rng(16)
-    sel = np.sort(rng.choice(n, 20000, replace=False)).astype(np.uint32)
+    sel = (np.arange(n, dtype=np.uint32) if sample is None
+           else np.sort(rng.choice(n, sample, replace=False)).astype(np.uint32))
     d = oracle.density_subset(p, sp, cs, ce, sel)
     rel = np.abs(meas[sel, 0] - d[sel, 0]) / d[sel, 0]
     assert rel.max() <= 1e-5, rel.max()
@@ -82,3 +82,29 @@ def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
     f = oracle.force_subset(p, meas, sp, sv, idx, cs, ce, fl)
     rows = idx[fl]
     assert normwise(acc[rows, :3], f[rows, :3]) <= 1e-5
+    assert np.all(np.isfinite(s.getArray(L.ARRAY_POSITION)))
+
+
+def test_dambreak_1m_500_steps_checked_every_50(device, oracle):
+    sc = S.dambreak3d(dx=5e-3)
+    p = sc.params
+    s = ParamSystem.from_state(p, sc.pos, sc.vel)
+    rng = np.random.default_rng(50)
+    for step in range(500):
+        if step % 50 == 0:
+            print(f"1M dam break, step {step}", flush=True)
+            check_step_on_own_state(s, oracle, p, s.getArray(L.ARRAY_POSITION), 100_000, rng)
+        else:
+            s.update()
+
+
+def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
+    sc = S.dambreak3d(dx=0.4 / 202, hash_order=L.HASH_MORTON, morton_xbits=4)
+    p = sc.params
+    print(f"16M dam break: {sc.num_fluid} fluid + {sc.meta['num_wall']} wall particles", flush=True)
+    s = ParamSystem.from_state(p, sc.pos, sc.vel)
+    rng = np.random.default_rng(16)
+    check_step_on_own_state(s, oracle, p, sc.pos, 20_000, rng)
+    s.run(198, use_graph=True)
+    print("16M dam break, step 199", flush=True)
+    check_step_on_own_state(s, oracle, p, s.getArray(L.ARRAY_POSITION), 20_000, rng)
