@@ -35,24 +35,13 @@
 #include "integrate_device.hpp"
 #include "kernels.hpp"
 #include "sph_device.hpp"
+#include "step_state.hpp"
 
 namespace cmag {
 
 struct BatchInst {
     uint32_t part0, part1;  // particle range (unsorted and sorted order alike)
     uint32_t cell0, cells;  // first global cell key, number of cells
-};
-
-// time state of one instance, advanced by batch_prologue_kernel
-struct BatchState {
-    float elapsed;
-    float time_shift, time_relax, wave_height;
-    float wave_height_inc;   // delta_time * 10^-3, computed on the host as Update() does
-    float t_w;               // wave time of the current step
-    float configure_height;  // wall height of this step's boundary configuration
-    int32_t is_config;       // boundary-configuration phase (after this step's switch)
-    int32_t configure_now;   // configure the walls in front of this step's hash
-    int32_t pad[3];
 };
 
 // the chunk [t0, t1) of tiles this workgroup runs: contiguous per workgroup,
@@ -66,35 +55,17 @@ __device__ __forceinline__ void tile_chunk(uint32_t num_tiles, uint32_t& t0, uin
 // PeristalsisSystem::Update's state machine (PeristalsisSystem.cpp:330-343),
 // one thread per instance; other systems only advance their clock
 __global__ __launch_bounds__(64) void batch_prologue_kernel(const cmag_params* __restrict__ params,
-                                                            BatchState* __restrict__ state, uint32_t m) {
-#pragma clang fp contract(off)
+                                                            StepState* __restrict__ state, uint32_t m) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
-    const cmag_params& P = params[k];
-    BatchState s = state[k];
-    s.configure_now = 0;
-    s.t_w = 0.0f;
-    if (P.system == CMAG_SYS_PERISTALSIS) {
-        if (s.is_config) {
-            s.time_shift += P.delta_time;
-            if (s.wave_height < P.amplitude) {
-                s.configure_now = 1;
-                s.configure_height = s.wave_height;
-                s.wave_height += s.wave_height_inc;
-            } else {
-                s.time_relax -= P.delta_time;
-                if (s.time_relax < 0.0f) s.is_config = !s.is_config;
-            }
-        }
-        s.t_w = s.is_config ? 0.0f : s.elapsed - s.time_shift;
-    }
-    s.elapsed += P.delta_time;
+    StepState s = state[k];
+    advance_step_state(params[k], s);
     state[k] = s;
 }
 
 __global__ __launch_bounds__(256) void batch_hash_kernel(const cmag_params* __restrict__ params,
                                                          const BatchInst* __restrict__ inst,
-                                                         const BatchState* __restrict__ state,
+                                                         const StepState* __restrict__ state,
                                                          const uint2* __restrict__ tiles, uint32_t num_tiles,
                                                          float4* __restrict__ pos, uint32_t* __restrict__ hash,
                                                          uint32_t* __restrict__ index) {
@@ -178,7 +149,7 @@ __global__ __launch_bounds__(256) void batch_force_kernel(const cmag_params* __r
 
 __global__ __launch_bounds__(256) void batch_integrate_kernel(const cmag_params* __restrict__ params,
                                                               const BatchInst* __restrict__ inst,
-                                                              const BatchState* __restrict__ state,
+                                                              const StepState* __restrict__ state,
                                                               const uint2* __restrict__ tiles, uint32_t num_tiles,
                                                               float4* __restrict__ pos, float4* __restrict__ vel,
                                                               float4* __restrict__ velLF,
@@ -218,7 +189,7 @@ struct cmag_batch {
 
     cmag_params* dP = nullptr;
     BatchInst* dI = nullptr;
-    BatchState* dS = nullptr;
+    StepState* dS = nullptr;
     uint2* dTiles = nullptr;
     uint32_t numTiles = 0, tilesFor = 0;  // tiles built for block size tilesFor
 
@@ -274,7 +245,7 @@ struct cmag_batch {
         CMAG_HIP_TRY(hipMalloc(&sortWs, sortWsBytes));
         CMAG_HIP_TRY(hipMalloc((void**)&dP, sizeof(cmag_params) * m));
         CMAG_HIP_TRY(hipMalloc((void**)&dI, sizeof(BatchInst) * m));
-        CMAG_HIP_TRY(hipMalloc((void**)&dS, sizeof(BatchState) * m));
+        CMAG_HIP_TRY(hipMalloc((void**)&dS, sizeof(StepState) * m));
         CMAG_HIP_TRY(hipMemcpyAsync(dP, P.data(), sizeof(cmag_params) * m, hipMemcpyHostToDevice, stream));
         CMAG_HIP_TRY(hipMemcpyAsync(dI, I.data(), sizeof(BatchInst) * m, hipMemcpyHostToDevice, stream));
         CMAG_TRY(reset_clocks());
@@ -286,14 +257,9 @@ struct cmag_batch {
     // elapsed time 0 and the Peristalsis configuration state of a fresh system
     // (PeristalsisSystem::Reset, PeristalsisSystem.cpp:213-237)
     int reset_clocks() {
-        std::vector<BatchState> S(m);
-        for (uint32_t k = 0; k < m; ++k) {
-            std::memset(&S[k], 0, sizeof(BatchState));
-            S[k].time_relax = 1000 * P[k].delta_time;
-            S[k].wave_height_inc = P[k].delta_time * powf(10.0f, -3.0f);
-            S[k].is_config = P[k].system == CMAG_SYS_PERISTALSIS && P[k].is_boundary_configuration ? 1 : 0;
-        }
-        CMAG_HIP_TRY(hipMemcpyAsync(dS, S.data(), sizeof(BatchState) * m, hipMemcpyHostToDevice, stream));
+        std::vector<StepState> S(m);
+        for (uint32_t k = 0; k < m; ++k) S[k] = initial_step_state(P[k], 0.0f);
+        CMAG_HIP_TRY(hipMemcpyAsync(dS, S.data(), sizeof(StepState) * m, hipMemcpyHostToDevice, stream));
         CMAG_HIP_TRY(hipStreamSynchronize(stream));
         return CMAG_OK;
     }
@@ -523,7 +489,7 @@ int snapshot(cmag_batch* b, Snapshot& s, bool restore) {
     if (!restore) {
         for (int k = 0; k < 4; ++k) {
             void* q = nullptr;
-            CMAG_HIP_TRY(hipMalloc(&q, k < 3 ? f4 : sizeof(BatchState) * b->m));
+            CMAG_HIP_TRY(hipMalloc(&q, k < 3 ? f4 : sizeof(StepState) * b->m));
             s.bufs.push_back(q);
         }
     }
@@ -534,7 +500,7 @@ int snapshot(cmag_batch* b, Snapshot& s, bool restore) {
     }
     void* dst = restore ? (void*)b->dS : s.bufs[3];
     const void* src = restore ? s.bufs[3] : (const void*)b->dS;
-    CMAG_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(BatchState) * b->m, hipMemcpyDeviceToDevice, b->stream));
+    CMAG_HIP_TRY(hipMemcpyAsync(dst, src, sizeof(StepState) * b->m, hipMemcpyDeviceToDevice, b->stream));
     CMAG_HIP_TRY(hipStreamSynchronize(b->stream));
     return CMAG_OK;
 }
@@ -647,7 +613,7 @@ void* cmag_batch_device_ptr(cmag_batch* b, int array) {
 
 int cmag_batch_elapsed_time(cmag_batch* b, uint32_t inst, float* out) {
     if (!b || inst >= b->m || !out) return fail(CMAG_E_INVALID, "bad batch, instance or output");
-    BatchState s;
+    StepState s;
     CMAG_HIP_TRY(hipStreamSynchronize(b->stream));
     CMAG_HIP_TRY(hipMemcpy(&s, b->dS + inst, sizeof(s), hipMemcpyDeviceToHost));
     *out = s.elapsed;

@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "integrate_device.hpp"
 #include "kernels.hpp"
+#include "step_state.hpp"
 
 namespace cmag {
 
@@ -20,24 +21,26 @@ __global__ __launch_bounds__(256) void integrate_kernel(const cmag_params P, flo
                                                          float4* __restrict__ velLF,
                                                          const float4* __restrict__ acc,
                                                          const float4* __restrict__ presF,
-                                                         float wave_time, uint32_t n) {
+                                                         float wave_time, const StepState* __restrict__ st,
+                                                         uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    integrate_body<SYS>(P, i, pos, vel, velLF, acc, presF, wave_time, P.is_boundary_configuration);
+    if (st) integrate_body<SYS>(P, i, pos, vel, velLF, acc, presF, st->t_w, st->is_config);
+    else integrate_body<SYS>(P, i, pos, vel, velLF, acc, presF, wave_time, P.is_boundary_configuration);
 }
 
 int launch_integrate(const cmag_params& P, float* pos, float* vel, float* velLF, const float* acc,
-                     const float* presF, float wave_time, uint32_t n, hipStream_t s) {
+                     const float* presF, float wave_time, uint32_t n, hipStream_t s, const StepState* st) {
     if (n == 0) return CMAG_OK;
     dim3 g(div_up(n, 256)), b(256);
     auto f4 = [](float* q) { return reinterpret_cast<float4*>(q); };
     auto c4 = [](const float* q) { return reinterpret_cast<const float4*>(q); };
     if (P.system == CMAG_SYS_DAMBREAK)
-        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_DAMBREAK>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, n);
+        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_DAMBREAK>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, st, n);
     else if (P.system == CMAG_SYS_POISEUILLE)
-        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_POISEUILLE>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, n);
+        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_POISEUILLE>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, st, n);
     else
-        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_PERISTALSIS>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, n);
+        hipLaunchKernelGGL(integrate_kernel<CMAG_SYS_PERISTALSIS>, g, b, 0, s, P, f4(pos), f4(vel), f4(velLF), c4(acc), c4(presF), wave_time, st, n);
     return check_launch("integrate_kernel");
 }
 
@@ -55,6 +58,38 @@ int launch_configure_boundary(const cmag_params& P, float* pos, float wave_heigh
     hipLaunchKernelGGL(configure_boundary_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, P,
                        reinterpret_cast<float4*>(pos), wave_height, n);
     return check_launch("configure_boundary_kernel");
+}
+
+__global__ __launch_bounds__(64) void step_prologue_kernel(const cmag_params P, StepState* __restrict__ state) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    StepState st = *state;
+    advance_step_state(P, st);
+    *state = st;
+}
+
+int launch_step_prologue(const cmag_params& P, StepState* state, hipStream_t s) {
+    hipLaunchKernelGGL(step_prologue_kernel, dim3(1), dim3(64), 0, s, P, state);
+    return check_launch("step_prologue_kernel");
+}
+
+// configureBoundaryD (PeristalsisKernel.cu:144-171) when this step's state
+// record asks for it; a no-op step otherwise
+__global__ __launch_bounds__(256) void configure_boundary_state_kernel(const cmag_params P, float4* __restrict__ pos,
+                                                                        const StepState* __restrict__ st,
+                                                                        uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !st->configure_now) return;
+    const float4 p = pos[i];
+    if (p.w > 0.0f || p.w < 0.0f)
+        pos[i] = make_float4(p.x, wall_y(P, p.x, p.w, st->configure_height, 0.0f), p.z, p.w);
+}
+
+int launch_configure_boundary_from_state(const cmag_params& P, float* pos, const StepState* state, uint32_t n,
+                                         hipStream_t s) {
+    if (n == 0) return CMAG_OK;
+    hipLaunchKernelGGL(configure_boundary_state_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, P,
+                       reinterpret_cast<float4*>(pos), state, n);
+    return check_launch("configure_boundary_state_kernel");
 }
 
 template <bool REMOVE>

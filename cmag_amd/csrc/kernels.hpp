@@ -11,6 +11,8 @@
 
 namespace cmag {
 
+struct StepState;
+
 int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, uint32_t* index,
@@ -55,10 +57,18 @@ int launch_viscous_force(const cmag_params& P, float* viscF, const float* measur
                          const float* sortedPos, const float* sortedVel, const uint32_t* index,
                          const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n,
                          hipStream_t s);
+// with a state record (step_state.hpp) the wave time and configuration phase
+// come from it on the device, else from wave_time and P
 int launch_integrate(const cmag_params& P, float* pos, float* vel, float* velLF, const float* acc,
-                     const float* presF, float wave_time, uint32_t n, hipStream_t s);
+                     const float* presF, float wave_time, uint32_t n, hipStream_t s,
+                     const StepState* state = nullptr);
 int launch_configure_boundary(const cmag_params& P, float* pos, float wave_height, uint32_t n,
                               hipStream_t s);
+// advance one system's state record by one step (one thread), then move the
+// walls to the record's configure_height when it says so
+int launch_step_prologue(const cmag_params& P, StepState* state, hipStream_t s);
+int launch_configure_boundary_from_state(const cmag_params& P, float* pos, const StepState* state, uint32_t n,
+                                         hipStream_t s);
 int launch_shift_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
 int launch_remove_right_boundary(const cmag_params& P, float* pos, uint32_t n, hipStream_t s);
 

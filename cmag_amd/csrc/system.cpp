@@ -13,6 +13,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "step_state.hpp"
 
 using namespace cmag;
 
@@ -37,8 +38,13 @@ struct cmag_system {
     float *gaccU = nullptr, *gaccS = nullptr, *gposS = nullptr, *gvelS = nullptr, *grho = nullptr;
 
     float elapsed = 0.0f;
-    // Peristalsis boundary-configuration state, PeristalsisSystem.cpp:330-343
-    float time_shift = 0.0f, time_relax = 0.0f, wave_height = 0.0f;
+    // Peristalsis boundary-configuration state (PeristalsisSystem.cpp:330-343):
+    // dState drives the kernels, hstate mirrors it on the host (same function,
+    // same arithmetic) for queries; state_dirty sends hstate to the device in
+    // front of the next step after the host changed it
+    StepState hstate{};
+    StepState* dState = nullptr;
+    bool state_dirty = true;
 
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
@@ -75,7 +81,7 @@ struct cmag_system {
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
-                        gaccU, gaccS, gposS, gvelS, grho};
+                        gaccU, gaccS, gposS, gvelS, grho, dState};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -91,6 +97,7 @@ struct cmag_system {
         if (P.system == CMAG_SYS_PERISTALSIS) {
             CMAG_HIP_TRY(hipMalloc((void**)&presF, f4));
             CMAG_HIP_TRY(hipMalloc((void**)&predicted, f4));
+            CMAG_HIP_TRY(hipMalloc((void**)&dState, sizeof(StepState)));
         }
         uint32_t** ub[] = {&hash, &index, &hashAlt, &indexAlt};
         for (uint32_t** b : ub) CMAG_HIP_TRY(hipMalloc((void**)b, u));
@@ -143,7 +150,7 @@ struct cmag_system {
     }
 
     // one update(): hash -> sort -> reorder/cellStart -> density -> forces -> integrate
-    int enqueue_step(float wave_time) {
+    int enqueue_step() {
         CMAG_TRY(enqueue_front());
         CMAG_TRY(mark(CMAG_STAGE_FORCE));
         if (P.system == CMAG_SYS_PERISTALSIS) {
@@ -154,37 +161,51 @@ struct cmag_system {
                                   stream, cellFirst));
         }
         CMAG_TRY(mark(CMAG_STAGE_INTEGRATE));
-        CMAG_TRY(launch_integrate(P, pos, vel, velLF, acc, presF, wave_time, n, stream));
+        CMAG_TRY(launch_integrate(P, pos, vel, velLF, acc, presF, 0.0f, n, stream, dState));
         CMAG_TRY(mark(CMAG_NUM_STAGES));
         if (timing) pending = true;
         return CMAG_OK;
     }
 
-    // PeristalsisSystem::Update's host state machine, PeristalsisSystem.cpp:330-343;
-    // returns the wave time t_w of the step
-    int peristalsis_prologue(float* t_w) {
-        *t_w = 0.0f;
-        if (P.system != CMAG_SYS_PERISTALSIS) return CMAG_OK;
-        if (P.is_boundary_configuration) {
-            time_shift += P.delta_time;
-            if (wave_height < P.amplitude) {
-                CMAG_TRY(launch_configure_boundary(P, pos, wave_height, n, stream));
-                wave_height += P.delta_time * powf(10.0f, -3.0f);
-            } else {
-                time_relax -= P.delta_time;
-                if (time_relax < 0) P.is_boundary_configuration = !P.is_boundary_configuration;
-            }
+    // one update() on the stream: for Peristalsis the state record advances on
+    // the device (PeristalsisSystem::Update, PeristalsisSystem.cpp:330-343) and
+    // the walls are configured when it says so, in front of the step
+    int enqueue_update() {
+        if (P.system == CMAG_SYS_PERISTALSIS) {
+            CMAG_TRY(launch_step_prologue(P, dState, stream));
+            CMAG_TRY(launch_configure_boundary_from_state(P, pos, dState, n, stream));
         }
-        *t_w = P.is_boundary_configuration ? 0.0f : elapsed - time_shift;
+        return enqueue_step();
+    }
+
+    // send the host's state record to the device if the host changed it
+    int upload_state() {
+        if (P.system != CMAG_SYS_PERISTALSIS || !state_dirty) return CMAG_OK;
+        hstate.elapsed = elapsed;
+        hstate.wave_height_inc = P.delta_time * powf(10.0f, -3.0f);
+        hstate.is_config = P.is_boundary_configuration ? 1 : 0;
+        CMAG_HIP_TRY(hipMemcpyAsync(dState, &hstate, sizeof(StepState), hipMemcpyHostToDevice, stream));
+        CMAG_HIP_TRY(hipStreamSynchronize(stream));
+        state_dirty = false;
         return CMAG_OK;
+    }
+
+    // the host side of one step: the mirror takes the step the device took
+    void advance_clock() {
+        if (P.system == CMAG_SYS_PERISTALSIS) {
+            advance_step_state(P, hstate);
+            P.is_boundary_configuration = hstate.is_config;
+            elapsed = hstate.elapsed;
+        } else {
+            elapsed += P.delta_time;
+        }
     }
 
     int update() {
         CMAG_TRY(collect());
-        float t_w = 0.0f;
-        CMAG_TRY(peristalsis_prologue(&t_w));
-        CMAG_TRY(enqueue_step(t_w));
-        elapsed += P.delta_time;
+        CMAG_TRY(upload_state());
+        CMAG_TRY(enqueue_update());
+        advance_clock();
         return CMAG_OK;
     }
 
@@ -201,7 +222,7 @@ struct cmag_system {
         const bool was_timing = timing;
         timing = false;  // no event nodes inside the graph
         CMAG_HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step(0.0f);
+        int rc = enqueue_update();
         timing = was_timing;
         hipError_t e = hipStreamEndCapture(stream, &g);
         if (rc != CMAG_OK) {
@@ -217,14 +238,16 @@ struct cmag_system {
     }
 
     int run(uint32_t steps, bool use_graph) {
-        if (!use_graph || P.system == CMAG_SYS_PERISTALSIS) {
+        if (!use_graph) {
             for (uint32_t k = 0; k < steps; ++k) CMAG_TRY(update());
             return CMAG_OK;
         }
+        CMAG_TRY(collect());
+        CMAG_TRY(upload_state());
         if (!graph_valid) CMAG_TRY(build_graph());
         for (uint32_t k = 0; k < steps; ++k) {
             CMAG_HIP_TRY(hipGraphLaunch(graph, stream));
-            elapsed += P.delta_time;
+            advance_clock();
         }
         return CMAG_OK;
     }
@@ -343,7 +366,7 @@ static int make_system(const cmag_params& P, uint32_t n, int ref_kind, cmag_syst
     s->n = n;
     s->numCells = cmag_params_num_cells(&P);
     s->key_bits = cmag_params_key_bits(&P);
-    s->time_relax = 1000 * P.delta_time;
+    s->hstate = initial_step_state(P, 0.0f);
     int rc = s->alloc();
     if (rc != CMAG_OK) {
         delete s;
@@ -423,10 +446,9 @@ int cmag_system_reset(cmag_system* s) {
     s->elapsed = 0.0f;
     if (s->ref_kind == CMAG_SYS_PERISTALSIS) {
         // PeristalsisSystem::Reset, PeristalsisSystem.cpp:213-237
-        s->time_shift = 0.0f;
-        s->time_relax = 1000 * s->P.delta_time;
-        s->wave_height = 0.0f;
         s->P.is_boundary_configuration = 1;
+        s->hstate = initial_step_state(s->P, 0.0f);
+        s->state_dirty = true;
         init_peristalsis(s->P, hpos.data(), n);
         CMAG_TRY(s->set_array(CMAG_ARRAY_POSITION, hpos.data(), 0, (int)n));
         CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITY, zero.data(), 0, (int)n));
@@ -465,6 +487,7 @@ int cmag_system_update_dt(cmag_system* s, float dt) {
     if (!(dt > 0.0f)) return fail(CMAG_E_INVALID, "update(dt): dt must be positive");
     if (dt != s->P.delta_time) {
         s->P.delta_time = dt;
+        s->state_dirty = true;
         s->invalidate_graph();
     }
     return s->update();
@@ -525,6 +548,7 @@ int cmag_system_set_params(cmag_system* s, const cmag_params* p) {
     if (p->system != s->P.system) return fail(CMAG_E_INVALID, "set_params may not change the system kind");
     s->P = *p;
     s->key_bits = cmag_params_key_bits(p);
+    s->state_dirty = true;
     s->invalidate_graph();
     return CMAG_OK;
 }

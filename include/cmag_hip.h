@@ -188,8 +188,9 @@ int cmag_system_reset(cmag_system* s);
 int cmag_system_update(cmag_system* s);
 /* update(dt): sets the time step, then update() */
 int cmag_system_update_dt(cmag_system* s, float dt);
-/* steps consecutive updates; DamBreak and Poiseuille replay one captured
- * HIP graph per step when use_graph is non-zero */
+/* steps consecutive updates, replaying one captured HIP graph per step when
+ * use_graph is non-zero (all three systems: the Peristalsis boundary-
+ * configuration state machine runs on the device) */
 int cmag_system_run(cmag_system* s, uint32_t steps, int use_graph);
 
 /* setArray(array, data, start, count), fluidSystem.cpp:321-355: count

@@ -129,6 +129,39 @@ def test_graph_replay_equals_eager(device):
     assert np.array_equal(a.getArray(L.ARRAY_POSITION), b.getArray(L.ARRAY_POSITION))
 
 
+def test_peristalsis_graph_through_configuration(device, oracle):
+    """a Peristalsis channel through its boundary-configuration phase (3-step
+    wall ramp, 1000 dt of relaxation, switch to the moving wave) replayed from
+    HIP graphs, with eager steps between replays, equals eager update() bit for
+    bit; the phase switch and clock follow the oracle's state machine and the
+    walls sit where the oracle puts them"""
+    sc = S.channel_peristalsis(1500, 0.3, 1.0, seed=302)
+    p = sc.params.copy()
+    p.is_boundary_configuration = 1
+    p.amplitude = np.float32(2.5 * p.delta_time * 1e-3)
+    a = ParamSystem.from_state(p, sc.pos, None)
+    b = ParamSystem.from_state(p, sc.pos, None)
+    sim = oracle.Simulation(p, sc.pos)
+    for chunk in (3, 997):
+        a.run(chunk, use_graph=False)
+        b.run(chunk, use_graph=True)
+        sim.update(chunk)
+        assert a.params.is_boundary_configuration == b.params.is_boundary_configuration == 1
+    a.run(10, use_graph=False)
+    b.run(4, use_graph=True)
+    b.update()
+    b.run(5, use_graph=True)
+    sim.update(10)
+    assert a.params.is_boundary_configuration == b.params.is_boundary_configuration == 0
+    assert sim.P.is_boundary_configuration == 0
+    for arr in (L.ARRAY_POSITION, L.ARRAY_VELOCITY, L.ARRAY_VELOCITYLEAPFROG):
+        assert np.array_equal(a.getArray(arr), b.getArray(arr))
+    assert a.getElapsedTime() == b.getElapsedTime() == np.float32(sim.elapsed)
+    pos = b.getArray(L.ARRAY_POSITION)
+    walls = sc.pos[:, 3] != 0.0
+    assert np.abs(pos[walls, 1] - sim.pos[walls, 1]).max() <= 1e-6 * np.abs(sim.pos[walls, 1]).max()
+
+
 def test_set_get_array_roundtrip_and_errors(device):
     s = DamBreakSystem((16, 16, 1), 1, (64, 32, 4), 1 / 64, False)
     s.reset()
