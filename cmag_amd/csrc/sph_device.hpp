@@ -116,9 +116,12 @@ struct Run {
 // the same loop whatever its x position -- lanes whose row splits at a
 // different x would otherwise serialise the wave.
 // PERIODIC = false compiles the image shifts out (DamBreak has none).
-template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
-__device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4& pi, const Cell& gp,
-                                             const Cells& cells, C&& cand) {
+//
+// walk_rows does the walk and hands each row's runs to row(a, b, qy, qz);
+// walk_stencil and walk_stencil_pf turn them into candidate calls.
+template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename R>
+__device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi, const Cell& gp,
+                                          const Cells& cells, R&& run_pair) {
     const int cc = P.cellcount;
     const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
     const uint32_t gxm = P.grid_size[0] - 1u, gym = P.grid_size[1] - 1u, gzm = P.grid_size[2] - 1u;
@@ -137,14 +140,6 @@ __device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4&
             if (DIM == 3) kz0 = dilated_dec(kz0, Mz);
         }
     }
-    auto run_pair = [&](const Run& a, const Run& b, float py, float pz) {
-        const uint32_t total = a.len + b.len;
-        for (uint32_t t = 0; t < total; ++t) {
-            const bool second = t >= a.len;
-            const uint32_t j = second ? b.s + (t - a.len) : a.s + t;
-            cand(j, second ? b.px : a.px, py, pz, second ? b.code : a.code);
-        }
-    };
     uint32_t kz = kz0;
     for (int dz = zlo; dz <= zhi; ++dz) {
         const int nz = gp.z + dz;
@@ -213,6 +208,52 @@ __device__ __forceinline__ float4 row4(const float4* __restrict__ base, uint32_t
     return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(base) + (j << 4));
 }
 
+template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
+__device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4& pi, const Cell& gp,
+                                             const Cells& cells, C&& cand) {
+    walk_rows<DIM, HASH, PERIODIC>(P, pi, gp, cells, [&](const Run& a, const Run& b, float py, float pz) {
+        const uint32_t total = a.len + b.len;
+        for (uint32_t t = 0; t < total; ++t) {
+            const bool second = t >= a.len;
+            const uint32_t j = second ? b.s + (t - a.len) : a.s + t;
+            cand(j, second ? b.px : a.px, py, pz, second ? b.code : a.code);
+        }
+    });
+}
+
+// walk_stencil with the candidate's sorted position loaded one candidate
+// ahead: cand(j, pos[j], valid, qx, qy, qz, code).  The load of candidate t + 1
+// is in flight while candidate t is evaluated, instead of each candidate
+// waiting for its own load.  The loop is unrolled by two with both loads and
+// both calls unconditional (the second index clamped to the run's last row,
+// and that call flagged valid = false when it lies past the run: the
+// candidate must then contribute nothing), so the two position registers
+// alternate without copies and the wait before each evaluation covers only
+// the older load.
+template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
+__device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const float4& pi, const Cell& gp,
+                                                const Cells& cells, const float4* __restrict__ pos, C&& cand) {
+    walk_rows<DIM, HASH, PERIODIC>(P, pi, gp, cells, [&](const Run& a, const Run& b, float py, float pz) {
+        const uint32_t total = a.len + b.len;
+        if (total == 0) return;
+        const uint32_t last = total - 1u;
+        const uint32_t off_b = b.s - a.len;
+        auto row = [&](uint32_t t) { return (t >= a.len ? off_b : a.s) + t; };
+        auto visit = [&](uint32_t t, const float4& pj, bool valid) {
+            const bool second = t >= a.len;
+            cand(row(t), pj, valid, second ? b.px : a.px, py, pz, second ? b.code : a.code);
+        };
+        float4 p0 = row4(pos, row(0));
+        for (uint32_t t = 0; t < total; t += 2) {
+            const uint32_t t1 = min(t + 1u, last);
+            const float4 p1 = row4(pos, row(t1));
+            visit(t, p0, true);
+            p0 = row4(pos, row(min(t + 2u, last)));
+            visit(t1, p1, t + 1u < total);
+        }
+    });
+}
+
 constexpr int NB_CAP = 16;
 constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
 
@@ -237,10 +278,11 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float inv_h = 1.0f / P.smoothing_radius;
     float sum = 0.0f;
-    walk_stencil<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, [&](uint32_t j, float px, float py,
-                                                                                  float pz, int) {
-        const float4 pj = row4(sortedPos, j);
-        const float r2 = norm2(px - pj.x, py - pj.y, pz - pj.z);
+    walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
+                                                         [&](uint32_t, const float4& pj, bool valid, float px,
+                                                             float py, float pz, int) {
+        // a candidate past the run sits at r^2 = 1e30: t = 0 and it adds +0
+        const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
         // (1 - q/2) clamped at 0 contributes nothing beyond 2h: no branch
         const float q = fsqrt(r2) * inv_h;
         const float t = fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f);
@@ -321,8 +363,10 @@ struct ForceSum {
 
     // pressure + artificial viscosity (DamBreak) or + Morris viscosity
     // (Poiseuille) of an in-range neighbour; one reciprocal square root and
-    // one reciprocal per pair (two with artificial viscosity)
-    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj, const float4& vj) {
+    // one reciprocal per pair (two with artificial viscosity); valid = false
+    // adds nothing
+    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj, const float4& vj,
+                                         bool valid = true) {
         const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
         const float r2 = norm2(rx, ry, rz);
         const float inv_d = frsq(r2);
@@ -336,12 +380,12 @@ struct ForceSum {
         if (SYS == CMAG_SYS_DAMBREAK) {
             const float vr = fmaf(vi.z - vj.z, rz, fmaf(vi.y - vj.y, ry, (vi.x - vj.x) * rx));
             if (vr < 0.0f) s += -av_k * vr * frcp((dens + mj.x) * (r2 + av_eps));
-            const float kk = -m * s * inv_d * temp;
+            const float kk = valid ? -m * s * inv_d * temp : 0.0f;
             fx = fmaf(kk, rx, fx); fy = fmaf(kk, ry, fy); fz = fmaf(kk, rz, fz);
         } else {
             const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
-            const float kk = -m * s * inv_d * temp;
-            const float kv = visc_k * inv_dens * inv_dj * inv_d * temp;
+            const float kk = valid ? -m * s * inv_d * temp : 0.0f;
+            const float kv = valid ? visc_k * inv_dens * inv_dj * inv_d * temp : 0.0f;
             fx += fmaf(kk, rx, kv * V.x);
             fy += fmaf(kk, ry, kv * V.y);
             fz += fmaf(kk, rz, kv * V.z);
@@ -356,7 +400,9 @@ struct ForceSum {
 // term would run for every candidate (some lane of the wave is almost always
 // in range), i.e. ~475 times per particle instead of ~74.  The list holds
 // CAP entries per lane (NB_CAP unless a caller picks another depth); when a
-// lane fills it, every active lane flushes.
+// lane fills it, every active lane flushes.  The flush takes entries two at a
+// time with all six row loads issued before either pair term (a lane with an
+// odd count repeats its last entry with the contribution zeroed).
 // nbl: this wave's neighbour lists in LDS, [CAP][64]
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
@@ -374,22 +420,25 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     int count = 0;
 
     auto flush = [&]() {
-        for (int k = 0; k < CAP; ++k) {
+        for (int k = 0; k < CAP; k += 2) {
             if (k >= count) break;
-            const uint32_t e = nbl[k][lane];
-            const uint32_t j = NbEntry::index(e);
-            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e)), row4(sortedPos, j),
-                   row4(measures, j), row4(sortedVel, j));
+            const uint32_t e0 = nbl[k][lane];
+            const uint32_t e1 = nbl[min(k + 1, count - 1)][lane];
+            const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
+            const float4 p0 = row4(sortedPos, j0), m0 = row4(measures, j0), v0 = row4(sortedVel, j0);
+            const float4 p1 = row4(sortedPos, j1), m1 = row4(measures, j1), v1 = row4(sortedVel, j1);
+            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e0)), p0, m0, v0);
+            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e1)), p1, m1, v1, k + 1 < count);
         }
         count = 0;
     };
 
-    walk_stencil<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, [&](uint32_t j, float px, float py,
-                                                                                  float pz, int code) {
-        const float4 pj = row4(sortedPos, j);
+    walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
+                                                         [&](uint32_t j, const float4& pj, bool valid, float px,
+                                                             float py, float pz, int code) {
         const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
         const float r2 = norm2(rx, ry, rz);
-        if (j != i) {
+        if (valid && j != i) {
             if (SYS == CMAG_SYS_DAMBREAK && pj.w != CMAG_DB_FLUID) {
                 // Lennard-Jones wall term: every wall particle of the stencil acts
                 F.wall(rx, ry, rz, r2);
@@ -471,11 +520,11 @@ __device__ __forceinline__ void peristalsis_force_body(
         count = 0;
     };
 
-    walk_stencil<DIM, HASH>(P, pi, gp, cells, [&](uint32_t j, float qx, float qy, float qz, int code) {
-        const float4 pj = row4(sortedPos, j);
+    walk_stencil_pf<DIM, HASH>(P, pi, gp, cells, sortedPos,
+                               [&](uint32_t j, const float4& pj, bool valid, float qx, float qy, float qz, int code) {
         const float rx = qx - pj.x, ry = qy - pj.y, rz = qz - pj.z;
         const float r2 = norm2(rx, ry, rz);
-        if (j != i && r2 < r2max) {
+        if (valid && j != i && r2 < r2max) {
             nbl[count][lane] = NbEntry::pack(j, code);
             ++count;
         }
