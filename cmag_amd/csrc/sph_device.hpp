@@ -221,15 +221,14 @@ __device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4&
     });
 }
 
-// walk_stencil with the candidate's sorted position loaded one candidate
-// ahead: cand(j, pos[j], valid, qx, qy, qz, code).  The load of candidate t + 1
-// is in flight while candidate t is evaluated, instead of each candidate
-// waiting for its own load.  The loop is unrolled by two with both loads and
-// both calls unconditional (the second index clamped to the run's last row,
-// and that call flagged valid = false when it lies past the run: the
-// candidate must then contribute nothing), so the two position registers
-// alternate without copies and the wait before each evaluation covers only
-// the older load.
+// walk_stencil with the candidates' sorted positions loaded ahead:
+// cand(j, pos[j], valid, qx, qy, qz, code).  The loop is unrolled by two; the
+// load of candidate t + 1 is issued before candidate t is evaluated and the
+// load for the next pass before candidate t + 1, so each wait covers loads
+// that were issued a candidate earlier.  Loads and calls are unconditional
+// (indices clamped to the run's last row, and the second call flagged
+// valid = false when it lies past the run: the candidate must then contribute
+// nothing), which keeps the loads out of divergent branches.
 template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
 __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const float4& pi, const Cell& gp,
                                                 const Cells& cells, const float4* __restrict__ pos, C&& cand) {
@@ -239,17 +238,20 @@ __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const floa
         const uint32_t last = total - 1u;
         const uint32_t off_b = b.s - a.len;
         auto row = [&](uint32_t t) { return (t >= a.len ? off_b : a.s) + t; };
-        auto visit = [&](uint32_t t, const float4& pj, bool valid) {
+        auto visit = [&](uint32_t t, uint32_t j, const float4& pj, bool valid) {
             const bool second = t >= a.len;
-            cand(row(t), pj, valid, second ? b.px : a.px, py, pz, second ? b.code : a.code);
+            cand(j, pj, valid, second ? b.px : a.px, py, pz, second ? b.code : a.code);
         };
-        float4 p0 = row4(pos, row(0));
+        uint32_t j0 = row(0);
+        float4 p0 = row4(pos, j0);
         for (uint32_t t = 0; t < total; t += 2) {
             const uint32_t t1 = min(t + 1u, last);
-            const float4 p1 = row4(pos, row(t1));
-            visit(t, p0, true);
-            p0 = row4(pos, row(min(t + 2u, last)));
-            visit(t1, p1, t + 1u < total);
+            const uint32_t j1 = row(t1);
+            const float4 p1 = row4(pos, j1);
+            visit(t, j0, p0, true);
+            j0 = row(min(t + 2u, last));
+            p0 = row4(pos, j0);
+            visit(t1, j1, p1, t + 1u < total);
         }
     });
 }
@@ -285,7 +287,8 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
         const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
         // (1 - q/2) clamped at 0 contributes nothing beyond 2h: no branch
         const float q = fsqrt(r2) * inv_h;
-        const float t = fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f);
+        // 1 - q/2 <= 1, so the clamp to [0, 1] is max(0, .) and folds into the fma
+        const float t = fminf(fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f), 1.0f);
         const float t2 = t * t;
         sum = fmaf(t2 * t2, fmaf(2.0f, q, 1.0f), sum);
     });
