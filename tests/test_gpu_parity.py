@@ -229,6 +229,51 @@ def test_force_parity(device, oracle, case):
     assert normwise(a[:, :3], want64[:, :3]) <= 1e-4
 
 
+def sparse_case(name, n, seed):
+    """n particles drawn from a small case: sparse cells, runs of length 0..3 and
+    odd candidate counts per row; channel_zorder puts the x-periodic channel on
+    plain Z order, whose rows split into three runs at the periodic seam"""
+    base = {"dam3d_sparse": "cfg_dam3d", "channel_sparse": "cfg_channel",
+            "channel_zorder": "cfg_channel", "peristalsis_sparse": "batch_peristalsis"}[name]
+    p, pos, vel = make_case(base)
+    if name == "channel_zorder":
+        p = p.copy()
+        p.hash_order = L.HASH_MORTON
+        p.morton_xbits = 0
+        S.finalize(p)
+    rng = np.random.default_rng(seed)
+    keep = np.sort(rng.choice(pos.shape[0], n, replace=False))
+    return p, pos[keep].copy(), vel[keep].copy()
+
+
+@pytest.mark.parametrize("name", ["dam3d_sparse", "channel_sparse", "channel_zorder", "peristalsis_sparse"])
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 65, 301])
+def test_sparse_and_ragged_walks(device, oracle, name, n):
+    """density and force on a few particles per stencil: single-candidate runs,
+    rows whose last candidate pair is half past the run, empty rows, lone
+    particles (density = self term, force = 0)"""
+    Stages, normwise = _gpu()
+    p, pos, vel = sparse_case(name, n, seed=1000 * n + len(name))
+    ref = oracle.run_pipeline(p, pos, vel)
+    sp, cs, ce = ref["sorted_pos"], ref["cell_start"], ref["cell_end"]
+    m = Stages.density(p, sp, cs, ce)
+    want = oracle.density(p, sp, cs, ce)
+    assert np.all(np.abs(m[:, 0] - want[:, 0]) <= 1e-5 * np.abs(want[:, 0]))
+    meas, sv, idx = ref["measures"], ref["sorted_vel"], ref["index"]
+    if p.system == L.SYS_PERISTALSIS:
+        pairs = [(Stages.pressure_force(p, meas, sp, idx, cs, ce), oracle.pressure_force(p, meas, sp, idx, cs, ce)),
+                 (Stages.viscous_force(p, meas, sp, sv, idx, cs, ce),
+                  oracle.viscous_force(p, meas, sp, sv, idx, cs, ce))]
+    else:
+        zero = np.zeros((n, 4), np.float32)
+        args = (meas, sp, sv, idx, cs, ce)
+        pairs = [(Stages.force(p, *args, acc_in=zero), oracle.force(p, *args, acc_in=zero))]
+    for a, f in pairs:
+        assert normwise(a[:, :3], f[:, :3]) <= 1e-5
+        if n == 1:
+            assert np.all(a[:, :3] == 0.0)
+
+
 @pytest.mark.parametrize("case", ["ref_peristalsis", "batch_peristalsis"])
 def test_peristalsis_pressure_and_viscous_parity(device, oracle, case):
     Stages, normwise = _gpu()
