@@ -18,8 +18,12 @@
 namespace cmag {
 
 constexpr int RS_NT = 256;
+// keys per thread: 16 (4096-key tiles) for large sorts; 4 (1024-key tiles) up
+// to RS_SMALL_N keys, where a pass is a handful of tiles and the in-tile
+// ranking rounds, not bandwidth, set its duration
 constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_NT * RS_ITEMS;  // 4096 keys per workgroup
+constexpr int RS_ITEMS_SMALL = 4;
+constexpr uint32_t RS_SMALL_N = 1u << 18;
 constexpr int SCAN_NT = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_NT * SCAN_ITEMS;
@@ -140,6 +144,7 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* 
 }
 
 // ---- radix passes ----------------------------------------------------------
+template <int ITEMS>
 __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
                                                          int shift, uint32_t* __restrict__ counts,
                                                          uint32_t numTiles) {
@@ -147,15 +152,15 @@ __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restri
     const int tid = threadIdx.x, lane = __lane_id();
     hist[tid] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * (uint32_t)RS_TILE;
-    uint32_t d[RS_ITEMS];
+    const uint32_t base = blockIdx.x * (uint32_t)(RS_NT * ITEMS);
+    uint32_t d[ITEMS];
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         uint32_t idx = base + k * RS_NT + tid;
         d[k] = idx < n ? (keys[idx] >> shift) & 255u : 0u;
     }
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         uint32_t idx = base + k * RS_NT + tid;
         bool valid = idx < n;
         uint64_t m = match_digit(d[k], valid);
@@ -166,12 +171,14 @@ __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restri
     counts[(size_t)tid * numTiles + blockIdx.x] = hist[tid];
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
     const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets, uint32_t numTiles) {
-    __shared__ uint32_t s_keys[RS_TILE];
-    __shared__ uint32_t s_vals[RS_TILE];
+    constexpr uint32_t TILE = RS_NT * ITEMS;
+    __shared__ uint32_t s_keys[TILE];
+    __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_start[256];
     __shared__ uint32_t s_run[256];
@@ -180,12 +187,12 @@ __global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
 
     const int tid = threadIdx.x, lane = __lane_id(), w = tid / 64;
     const uint32_t tile = blockIdx.x;
-    const uint32_t base = tile * (uint32_t)RS_TILE;
-    const uint32_t cnt = min((uint32_t)RS_TILE, n - base);
+    const uint32_t base = tile * TILE;
+    const uint32_t cnt = min(TILE, n - base);
 
-    uint32_t rk[RS_ITEMS], rv[RS_ITEMS];
+    uint32_t rk[ITEMS], rv[ITEMS];
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         uint32_t idx = base + k * RS_NT + tid;
         rk[k] = idx < n ? keys_in[idx] : 0u;
         rv[k] = idx < n ? vals_in[idx] : 0u;
@@ -199,7 +206,7 @@ __global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
 #pragma unroll
         for (int ww = 0; ww < RS_NT / 64; ++ww) s_wcnt[ww][tid] = 0;
         __syncthreads();
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
     }
 
 #pragma unroll
-    for (int k = 0; k < RS_ITEMS; ++k) {
+    for (int k = 0; k < ITEMS; ++k) {
         uint32_t p = k * RS_NT + tid;
         if (p < cnt) {
             uint32_t key = s_keys[p];
@@ -255,7 +262,8 @@ int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s)
     return check_launch("copy_u32_kernel");
 }
 
-static uint32_t num_tiles(uint32_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+static uint32_t tile_keys(uint32_t n) { return RS_NT * (n <= RS_SMALL_N ? RS_ITEMS_SMALL : RS_ITEMS); }
+static uint32_t num_tiles(uint32_t n) { return (uint32_t)(((uint64_t)n + tile_keys(n) - 1) / tile_keys(n)); }
 
 size_t radix_sort_workspace_bytes(uint32_t n) {
     size_t len = (size_t)256 * num_tiles(n);
@@ -277,12 +285,22 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_
     uint32_t* offsets = counts + len;
     uint32_t* scan_ws = offsets + len;
     uint32_t *kin = keys, *vin = vals, *kout = alt_keys, *vout = alt_vals;
+    const bool small = n <= RS_SMALL_N;
     for (int shift = 0; shift < key_bits; shift += 8) {
-        hipLaunchKernelGGL(rs_hist_kernel, dim3(tiles), dim3(RS_NT), 0, s, kin, n, shift, counts, tiles);
+        if (small)
+            hipLaunchKernelGGL(rs_hist_kernel<RS_ITEMS_SMALL>, dim3(tiles), dim3(RS_NT), 0, s, kin, n, shift,
+                               counts, tiles);
+        else
+            hipLaunchKernelGGL(rs_hist_kernel<RS_ITEMS>, dim3(tiles), dim3(RS_NT), 0, s, kin, n, shift, counts,
+                               tiles);
         CMAG_TRY(check_launch("rs_hist_kernel"));
         CMAG_TRY(exclusive_scan_u32(counts, offsets, len, scan_ws, s));
-        hipLaunchKernelGGL(rs_scatter_kernel, dim3(tiles), dim3(RS_NT), 0, s, kin, vin, kout, vout, n,
-                           shift, counts, offsets, tiles);
+        if (small)
+            hipLaunchKernelGGL(rs_scatter_kernel<RS_ITEMS_SMALL>, dim3(tiles), dim3(RS_NT), 0, s, kin, vin, kout,
+                               vout, n, shift, counts, offsets, tiles);
+        else
+            hipLaunchKernelGGL(rs_scatter_kernel<RS_ITEMS>, dim3(tiles), dim3(RS_NT), 0, s, kin, vin, kout, vout,
+                               n, shift, counts, offsets, tiles);
         CMAG_TRY(check_launch("rs_scatter_kernel"));
         std::swap(kin, kout);
         std::swap(vin, vout);

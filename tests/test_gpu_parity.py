@@ -144,6 +144,8 @@ SORT_CASES = [
     (1_000_003, 24, "random"), (1 << 20, 32, "random"), (50_000, 16, "allsame"), (123_457, 9, "random"),
     # a few tiles of 4096 keys, full 32-bit keys, partial last tiles
     (20_000, 9, "dups"), (32_768, 32, "random"), (32_768, 17, "reversed"), (32_769, 24, "random"),
+    # either side of the switch from 1024-key to 4096-key tiles at 2^18 keys
+    (1 << 18, 24, "random"), ((1 << 18) + 1, 24, "dups"), (1025, 11, "reversed"),
 ]
 
 
