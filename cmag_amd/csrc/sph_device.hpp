@@ -267,6 +267,44 @@ struct NbEntry {
     __device__ static __forceinline__ int code(uint32_t e) { return (int)(e >> NB_SHIFT_BITS); }
 };
 
+// A per-lane queue of CAP (a power of two) neighbour entries in LDS, one
+// [CAP][64] block per wave.  Entries leave in the order they came, so pair
+// terms are summed in walk order whatever the queue depth and drain sizes.
+// The walk pushes; when any lane's queue is full every active lane drains its
+// NB_DRAIN oldest entries (fewer if it has fewer), which keeps all lanes busy
+// in the pair loop instead of waiting for the fullest one; a final drain
+// empties the queues.
+constexpr int NB_DRAIN = 2;
+
+template <int CAP>
+struct NbQueue {
+    static_assert((CAP & (CAP - 1)) == 0 && CAP >= 2, "queue depth: a power of two");
+    uint32_t (*q)[64];
+    int lane;
+    int head = 0, count = 0;
+    __device__ __forceinline__ NbQueue(uint32_t (*q_)[64], int lane_) : q(q_), lane(lane_) {}
+    __device__ __forceinline__ void push(uint32_t e) {
+        q[(head + count) & (CAP - 1)][lane] = e;
+        ++count;
+    }
+    __device__ __forceinline__ bool full() const { return count == CAP; }
+    // f(e0, e1, second): the oldest entries two at a time; a lane with an odd
+    // number left repeats its last entry with second = false (contributing
+    // nothing), so both entries' rows can be loaded before either is used
+    template <typename F>
+    __device__ __forceinline__ void drain(int upto, F&& f) {
+        for (int k = 0; k < upto; k += 2) {
+            if (k >= count) break;
+            const uint32_t e0 = q[(head + k) & (CAP - 1)][lane];
+            const uint32_t e1 = q[(head + min(k + 1, count - 1)) & (CAP - 1)][lane];
+            f(e0, e1, k + 1 < count);
+        }
+        const int d = min(upto, count);
+        head = (head + d) & (CAP - 1);
+        count -= d;
+    }
+};
+
 // ---------------------------------------------------------------------------
 template <int SYS, int DIM, int HASH, typename Cells>
 __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, float4* __restrict__ measures,
@@ -401,12 +439,11 @@ struct ForceSum {
 // the in-range ones to a per-lane list in LDS; phase 2 evaluates the pair
 // terms of the listed neighbours.  Without the split, the ~50-instruction pair
 // term would run for every candidate (some lane of the wave is almost always
-// in range), i.e. ~475 times per particle instead of ~74.  The list holds
-// CAP entries per lane (NB_CAP unless a caller picks another depth); when a
-// lane fills it, every active lane flushes.  The flush takes entries two at a
-// time with all six row loads issued before either pair term (a lane with an
-// odd count repeats its last entry with the contribution zeroed).
-// nbl: this wave's neighbour lists in LDS, [CAP][64]
+// in range), i.e. ~475 times per particle instead of ~74.  The list is an
+// NbQueue of CAP entries per lane (NB_CAP unless a caller picks another
+// depth); pair terms take entries two at a time with all six row loads issued
+// before either term.
+// nbl: this wave's queues in LDS, [CAP][64]
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -420,20 +457,13 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     ForceSum<SYS> F(P, pi, sortedVel[i], measures[i]);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
-    int count = 0;
-
-    auto flush = [&]() {
-        for (int k = 0; k < CAP; k += 2) {
-            if (k >= count) break;
-            const uint32_t e0 = nbl[k][lane];
-            const uint32_t e1 = nbl[min(k + 1, count - 1)][lane];
-            const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
-            const float4 p0 = row4(sortedPos, j0), m0 = row4(measures, j0), v0 = row4(sortedVel, j0);
-            const float4 p1 = row4(sortedPos, j1), m1 = row4(measures, j1), v1 = row4(sortedVel, j1);
-            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e0)), p0, m0, v0);
-            F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e1)), p1, m1, v1, k + 1 < count);
-        }
-        count = 0;
+    NbQueue<CAP> nq(nbl, lane);
+    auto pairs = [&](uint32_t e0, uint32_t e1, bool second) {
+        const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
+        const float4 p0 = row4(sortedPos, j0), m0 = row4(measures, j0), v0 = row4(sortedVel, j0);
+        const float4 p1 = row4(sortedPos, j1), m1 = row4(measures, j1), v1 = row4(sortedVel, j1);
+        F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e0)), p0, m0, v0);
+        F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e1)), p1, m1, v1, second);
     };
 
     walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
@@ -446,13 +476,12 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                 // Lennard-Jones wall term: every wall particle of the stencil acts
                 F.wall(rx, ry, rz, r2);
             } else if (r2 < r2max) {
-                nbl[count][lane] = NbEntry::pack(j, code);
-                ++count;
+                nq.push(NbEntry::pack(j, code));
             }
         }
-        if (__ballot(count == CAP)) flush();
+        if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
     });
-    flush();
+    nq.drain(CAP, pairs);
     const uint32_t k = index[i];
     if (k < n) acc[k] = make_float4(F.fx + F.wx, F.fy + F.wy, F.fz + F.wz, 0.0f);
 }
@@ -476,8 +505,9 @@ struct PeriSum {
         gc = P.grad_coeff;
         visc_k = m * (P.mu + P.mu);
     }
-    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj,
-                                         const float4* __restrict__ sortedVel, uint32_t j) {
+    // valid = false adds nothing
+    __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj, const float4& vj,
+                                         bool valid = true) {
         const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
         const float r2 = norm2(rx, ry, rz);
         const float inv_d = frsq(r2);
@@ -488,12 +518,11 @@ struct PeriSum {
         const float temp = gc * fmaf(t2, t2, -(t2 * t) * fmaf(2.0f, qq, 1.0f));
         const float inv_dj = frcp(mj.x);
         if (WANT_PRES) {
-            const float kk = -m * temp * fmaf(mj.y * inv_dj, inv_dj, pterm_i) * inv_d;
+            const float kk = valid ? -m * temp * fmaf(mj.y * inv_dj, inv_dj, pterm_i) * inv_d : 0.0f;
             px = fmaf(kk, rx, px); py = fmaf(kk, ry, py); pz = fmaf(kk, rz, pz);
         }
         if (WANT_VISC) {
-            const float4 vj = row4(sortedVel, j);
-            const float kv = visc_k * temp * inv_dens * inv_dj * inv_d;
+            const float kv = valid ? visc_k * temp * inv_dens * inv_dj * inv_d : 0.0f;
             vx = fmaf(kv, vi.x - vj.x, vx); vy = fmaf(kv, vi.y - vj.y, vy); vz = fmaf(kv, vi.z - vj.z, vz);
         }
     }
@@ -511,29 +540,26 @@ __device__ __forceinline__ void peristalsis_force_body(
                                     measures[i]);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
-    int count = 0;
-
-    auto flush = [&]() {
-        for (int k = 0; k < CAP; ++k) {
-            if (k >= count) break;
-            const uint32_t e = nbl[k][lane];
-            const uint32_t j = NbEntry::index(e);
-            F.pair(image_target(P, pi, NbEntry::code(e)), row4(sortedPos, j), row4(measures, j), sortedVel, j);
-        }
-        count = 0;
+    NbQueue<CAP> nq(nbl, lane);
+    const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto pairs = [&](uint32_t e0, uint32_t e1, bool second) {
+        const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
+        const float4 p0 = row4(sortedPos, j0), m0 = row4(measures, j0);
+        const float4 v0 = WANT_VISC ? row4(sortedVel, j0) : zero4;
+        const float4 p1 = row4(sortedPos, j1), m1 = row4(measures, j1);
+        const float4 v1 = WANT_VISC ? row4(sortedVel, j1) : zero4;
+        F.pair(image_target(P, pi, NbEntry::code(e0)), p0, m0, v0);
+        F.pair(image_target(P, pi, NbEntry::code(e1)), p1, m1, v1, second);
     };
 
     walk_stencil_pf<DIM, HASH>(P, pi, gp, cells, sortedPos,
                                [&](uint32_t j, const float4& pj, bool valid, float qx, float qy, float qz, int code) {
         const float rx = qx - pj.x, ry = qy - pj.y, rz = qz - pj.z;
         const float r2 = norm2(rx, ry, rz);
-        if (valid && j != i && r2 < r2max) {
-            nbl[count][lane] = NbEntry::pack(j, code);
-            ++count;
-        }
-        if (__ballot(count == CAP)) flush();
+        if (valid && j != i && r2 < r2max) nq.push(NbEntry::pack(j, code));
+        if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
     });
-    flush();
+    nq.drain(CAP, pairs);
     const uint32_t k = index[i];
     if (k >= n) return;
     if (WANT_VISC) viscF[k] = make_float4(F.vx, F.vy, F.vz, 0.0f);
