@@ -71,6 +71,7 @@ __device__ __forceinline__ float eos(const cmag_params& P, float dens) {
 //              cellEnd of the run (empty cells hold 0xffffffff)
 //   FirstCells compact offsets first[c] (cells before c), one pair of loads
 struct RefCells {
+    static constexpr bool SUB = false;
     const uint32_t* __restrict__ cellStart;
     const uint32_t* __restrict__ cellEnd;
     uint32_t n;
@@ -90,11 +91,59 @@ struct RefCells {
 };
 
 struct FirstCells {
+    static constexpr bool SUB = false;
     const uint32_t* __restrict__ first;
     __device__ __forceinline__ bool range(uint32_t k0, uint32_t k1, uint32_t& s, uint32_t& e) const {
         s = first[k0];
         e = first[k1 + 1];
         return s < e;
+    }
+};
+
+// ---- sub-cell walk order (DamBreak) ---------------------------------------------
+// Inside each cell the particles can be kept in a second, internal order: by
+// bucket, then in reference order, with buckets 0..3 the fluid particles of
+// each quarter of the cell along x and bucket 4 the wall particles.  A
+// target in quarter q of its cell needs, of the stencil's first cell along
+// x, only the fluid of quarters >= q (the rest lies more than cellcount cells
+// away, beyond 2h), and of the last cell only the quarters <= q.  Wall
+// particles sit in their own bucket and are always walked: DamBreak's
+// Lennard-Jones term acts on every wall particle of the stencil.
+// x_quarter is the target's / particle's quarter, from the quotient that
+// grid_pos floors (the DIV_CELL rule).
+__device__ __forceinline__ int x_quarter(const cmag_params& P, float x) {
+#pragma clang fp contract(off)
+    const float u = (x - P.world_origin[0]) / P.cell_size[0];
+    return min(3, (int)((u - floorf(u)) * 4.0f));
+}
+
+// SubCells: cell offsets first[] as FirstCells plus, per cell, the running
+// counts pre[c] = (#bucket 0, #buckets <= 1, #buckets <= 2, #fluid); cells
+// without particles hold zeros
+struct SubCells {
+    static constexpr bool SUB = true;
+    const uint32_t* __restrict__ first;
+    const uint4* __restrict__ pre;
+    __device__ __forceinline__ static uint32_t through(const uint4& c, int q) {
+        return q == 0 ? c.x : q == 1 ? c.y : q == 2 ? c.z : c.w;
+    }
+    // the run k0..k1 trimmed for a target in quarter q: lo (k0 is the
+    // stencil's first cell along x) drops k0's fluid quarters < q, hi (k1 is
+    // the last) ends at k1's quarter q and returns k1's walls as [ws, we)
+    __device__ __forceinline__ void range_sub(uint32_t k0, uint32_t k1, bool lo, bool hi, int q, uint32_t& s,
+                                              uint32_t& e, uint32_t& ws, uint32_t& we) const {
+        s = first[k0];
+        if (lo && q > 0) s += through(pre[k0], q - 1);
+        ws = we = 0;
+        if (hi) {
+            const uint32_t f1 = first[k1];
+            const uint4 c1 = pre[k1];
+            e = f1 + through(c1, q);
+            ws = f1 + c1.w;
+            we = first[k1 + 1];
+        } else {
+            e = first[k1 + 1];
+        }
     }
 };
 
@@ -124,6 +173,7 @@ __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi
                                           const Cells& cells, R&& run_pair) {
     const int cc = P.cellcount;
     const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
+    const int sub = Cells::SUB ? x_quarter(P, pi.x) : 0;
     const uint32_t gxm = P.grid_size[0] - 1u, gym = P.grid_size[1] - 1u, gzm = P.grid_size[2] - 1u;
     const uint32_t Mx = P.morton_mask[0], My = P.morton_mask[1], Mz = P.morton_mask[2];
     const bool perx = PERIODIC && (P.periodic_mask & 1), pery = PERIODIC && (P.periodic_mask & 2),
@@ -156,24 +206,35 @@ __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi
                                      : (kz | ky);
             Run a{0u, 0u, pi.x, 13}, b{0u, 0u, pi.x, 13};
             bool have_a = false;
-            auto emit = [&](uint32_t k0, uint32_t k1, float px, int code) {
-                uint32_t s, e;
-                if (!cells.range(k0, k1, s, e)) return;
+            auto push = [&](uint32_t s, uint32_t e, float px, int code) {
                 const Run r{s, e - s, px, code};
                 if (!have_a) {
                     a = r;
                     have_a = true;
                 } else if (b.len == 0) {
                     b = r;
-                } else {  // a third run in one row (periodic seam with plain Z order)
+                } else {  // a third run in one row (periodic seam with plain Z order, or walls)
                     run_pair(a, b, py, pz);
                     a = r;
                     b.len = 0;
                 }
             };
+            // lo / hi: the run starts at the stencil's first cell / ends at its last
+            auto emit = [&](uint32_t k0, uint32_t k1, float px, int code, bool lo, bool hi) {
+                uint32_t s, e;
+                if constexpr (Cells::SUB) {
+                    uint32_t ws, we;
+                    cells.range_sub(k0, k1, lo, hi, sub, s, e, ws, we);
+                    if (s < e) push(s, e, px, code);
+                    if (ws < we) push(ws, we, px, code);
+                } else {
+                    if (cells.range(k0, k1, s, e)) push(s, e, px, code);
+                }
+            };
             uint32_t k0 = 0, k1 = 0;
             float run_px = pi.x;
             int run_sx = 0;
+            bool run_lo = true;
             uint32_t kx = kx0;
             for (int dx = -cc; dx <= cc; ++dx) {
                 const int nx = gp.x + dx;
@@ -187,14 +248,15 @@ __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi
                 } else if (key == k1 + 1u && sx == run_sx) {
                     k1 = key;
                 } else {
-                    emit(k0, k1, run_px, (run_sx + 1) + code_yz);
+                    emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, false);
                     k0 = k1 = key;
                     run_px = px;
                     run_sx = sx;
+                    run_lo = false;
                 }
                 if (HASH == CMAG_HASH_MORTON) kx = dilated_inc(kx, Mx);
             }
-            emit(k0, k1, run_px, (run_sx + 1) + code_yz);
+            emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, true);
             run_pair(a, b, py, pz);
             if (HASH == CMAG_HASH_MORTON) ky = dilated_inc(ky, My);
         }
@@ -306,9 +368,13 @@ struct NbQueue {
 };
 
 // ---------------------------------------------------------------------------
+// measuresR / w2r (optional): also write the result to measuresR[w2r[i]],
+// the reference-order row of sorted row i when the walk runs in sub-cell order
 template <int SYS, int DIM, int HASH, typename Cells>
 __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, float4* __restrict__ measures,
-                                             const float4* __restrict__ sortedPos, const Cells& cells) {
+                                             const float4* __restrict__ sortedPos, const Cells& cells,
+                                             float4* __restrict__ measuresR = nullptr,
+                                             const uint32_t* __restrict__ w2r = nullptr) {
     const float4 pi = sortedPos[i];
     float2* m2 = reinterpret_cast<float2*>(measures);
     if (SYS == CMAG_SYS_POISEUILLE && pi.w == CMAG_PO_WALL) {
@@ -331,10 +397,13 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
         sum = fmaf(t2 * t2, fmaf(2.0f, q, 1.0f), sum);
     });
     const float dens = sum * P.kernel_coeff * P.particle_mass;
-    if (SYS == CMAG_SYS_PERISTALSIS)
+    if (SYS == CMAG_SYS_PERISTALSIS) {
         measures[i] = make_float4(dens, eos(P, dens), 0.0f, pi.w);
-    else
-        m2[2 * (size_t)i] = make_float2(dens, eos(P, dens));
+    } else {
+        const float2 m = make_float2(dens, eos(P, dens));
+        m2[2 * (size_t)i] = m;
+        if (measuresR) reinterpret_cast<float2*>(measuresR)[2 * (size_t)w2r[i]] = m;
+    }
 }
 
 // getVelocityDiff, poiseuilleFlowKernel.cu:184-208

@@ -36,6 +36,13 @@ struct cmag_system {
     size_t sortWsBytes = 0;
     // adjoint scratch (sorted order), allocated by the first vjp_step
     float *gaccU = nullptr, *gaccS = nullptr, *gposS = nullptr, *gvelS = nullptr, *grho = nullptr;
+    // sub-cell walk order (DamBreak): the neighbour passes run over copies of
+    // the sorted rows in bucket order inside each cell; measures also land in
+    // the reference order, so every array a caller sees keeps its meaning
+    bool subwalk = false;
+    float *posW = nullptr, *velW = nullptr, *measW = nullptr;
+    uint32_t *indexW = nullptr, *w2r = nullptr, *preW = nullptr;
+    uint8_t* bucketR = nullptr;
 
     float elapsed = 0.0f;
     // Peristalsis boundary-configuration state (PeristalsisSystem.cpp:330-343):
@@ -81,7 +88,7 @@ struct cmag_system {
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
-                        gaccU, gaccS, gposS, gvelS, grho, dState};
+                        gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, measW, indexW, w2r, preW, bucketR};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -119,6 +126,34 @@ struct cmag_system {
 
     int passes() const { return (key_bits + 7) / 8; }
 
+    // the sub-cell walk needs DamBreak's DIV_CELL cells, no periodic images,
+    // rows whose first and last cells are distinct, and a stencil that reaches
+    // 2h along x (what it drops from the outer cells lies beyond cellcount cells)
+    bool subwalk_eligible() const {
+        return P.system == CMAG_SYS_DAMBREAK && P.gridpos_rule == CMAG_GRIDPOS_DIV_CELL && P.periodic_mask == 0 &&
+               P.grid_size[0] >= 2u * (uint32_t)P.cellcount + 2u &&
+               (double)P.cellcount * P.cell_size[0] >= 2.0 * P.smoothing_radius;
+    }
+
+    int set_subwalk(bool on) {
+        if (on && !subwalk_eligible())
+            return fail(CMAG_E_INVALID, "the sub-cell walk order needs a DamBreak system with DIV_CELL cells, no "
+                                        "periodic images, more than 2*cellcount+1 cells along x and "
+                                        "cellcount * cell_size.x >= 2h");
+        if (on && !posW) {
+            const size_t f4 = sizeof(float) * 4 * (size_t)(n ? n : 1);
+            const size_t u = sizeof(uint32_t) * (size_t)(n ? n : 1);
+            for (float** b : {&posW, &velW, &measW}) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
+            for (uint32_t** b : {&indexW, &w2r}) CMAG_HIP_TRY(hipMalloc((void**)b, u));
+            // padded to whole 16-byte words: the move kernel reads buckets 16 at a time
+            CMAG_HIP_TRY(hipMalloc((void**)&bucketR, ((size_t)n + 31) & ~(size_t)15));
+            CMAG_HIP_TRY(hipMalloc((void**)&preW, sizeof(uint32_t) * 4 * (size_t)numCells));
+        }
+        if (on != subwalk) invalidate_graph();
+        subwalk = on;
+        return CMAG_OK;
+    }
+
     // hash -> sort -> reorder/cellStart -> density: the neighbour structure
     // and densities of the current state
     int enqueue_front() {
@@ -144,8 +179,14 @@ struct cmag_system {
         CMAG_TRY(launch_reorder(cellStart, cellEnd, sortedPos, sortedVel, hash, index, pos, velLF, n,
                                 numCells, stream));
         CMAG_TRY(launch_cell_offsets(cellStart, cellEnd, cellFirst, numCells, n, scanWs, stream));
+        if (subwalk)
+            CMAG_TRY(launch_subcell_build(P, hash, index, sortedPos, sortedVel, cellFirst, n, numCells, preW, bucketR,
+                                          posW, velW, indexW, w2r, stream));
         CMAG_TRY(mark(CMAG_STAGE_DENSITY));
-        CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
+        if (subwalk)
+            CMAG_TRY(launch_density_sub(P, measW, measures, w2r, posW, cellFirst, preW, n, stream));
+        else
+            CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
         return CMAG_OK;
     }
 
@@ -156,6 +197,8 @@ struct cmag_system {
         if (P.system == CMAG_SYS_PERISTALSIS) {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
                                                cellStart, cellEnd, n, stream, cellFirst));
+        } else if (subwalk) {
+            CMAG_TRY(launch_force_sub(P, acc, measW, posW, velW, indexW, cellFirst, preW, n, stream));
         } else {
             CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
                                   stream, cellFirst));
@@ -368,6 +411,7 @@ static int make_system(const cmag_params& P, uint32_t n, int ref_kind, cmag_syst
     s->key_bits = cmag_params_key_bits(&P);
     s->hstate = initial_step_state(P, 0.0f);
     int rc = s->alloc();
+    if (rc == CMAG_OK && s->subwalk_eligible()) rc = s->set_subwalk(true);
     if (rc != CMAG_OK) {
         delete s;
         return rc;
@@ -550,8 +594,17 @@ int cmag_system_set_params(cmag_system* s, const cmag_params* p) {
     s->key_bits = cmag_params_key_bits(p);
     s->state_dirty = true;
     s->invalidate_graph();
+    if (s->subwalk && !s->subwalk_eligible()) s->subwalk = false;
     return CMAG_OK;
 }
+
+int cmag_system_set_subcell_walk(cmag_system* s, int on) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_TRY(s->collect());
+    return s->set_subwalk(on != 0);
+}
+
+int cmag_system_subcell_walk(const cmag_system* s) { return s && s->subwalk ? 1 : 0; }
 
 int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream) {
     if (!s) return fail(CMAG_E_INVALID, "null system");

@@ -125,6 +125,15 @@ class SystemBase:
         call("cmag_system_stage_times", self._h, ms, cnt, 6)
         return {name: (ms[k], cnt[k]) for k, name in enumerate(L.STAGES)}
 
+    def set_subcell_walk(self, on: bool) -> None:
+        """DamBreak neighbour passes over the internal sub-cell order (on by
+        default where it applies; results agree with the plain walk to rounding)"""
+        call("cmag_system_set_subcell_walk", self._h, 1 if on else 0)
+
+    @property
+    def subcell_walk(self) -> bool:
+        return bool(lib().cmag_system_subcell_walk(self._h))
+
     def synchronize(self) -> None:
         call("cmag_system_synchronize", self._h)
 

@@ -212,6 +212,19 @@ int cmag_system_get_params(const cmag_system* s, cmag_params* out);
  * not change the particle count or the grid */
 int cmag_system_set_params(cmag_system* s, const cmag_params* p);
 int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream);
+/* Neighbour-walk order of a DamBreak system (no counterpart in the reference,
+ * whose walk is the only order): with on != 0 the density and force passes
+ * run over an internal copy of the sorted rows ordered inside each cell by
+ * x-quarter (fluid) with the wall particles last, and skip the quarters of the
+ * stencil's outer cells that lie beyond 2h.  Every array a caller reads
+ * (hash, index, cell ranges, sorted rows, measures in reference order, forces)
+ * keeps its meaning; density and force sums run in another order, so they
+ * agree with the plain walk to rounding, not bit for bit.  On by default for
+ * the systems that qualify (DamBreak, DIV_CELL cells, no periodic images,
+ * more than 2*cellcount+1 cells along x, cellcount * cell_size.x >= 2h);
+ * CMAG_E_INVALID for others. */
+int cmag_system_set_subcell_walk(cmag_system* s, int on);
+int cmag_system_subcell_walk(const cmag_system* s);
 int cmag_system_synchronize(cmag_system* s);
 
 /* stage timing (no reference counterpart): with timing on, update() records

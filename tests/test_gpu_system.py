@@ -112,6 +112,51 @@ def test_dam3d_config_system_matches_oracle(device, oracle, order):
     assert rel_field(s.getArray(L.ARRAY_POSITION), sim.pos) <= 1e-6
 
 
+@pytest.mark.parametrize("order,xbits", [(L.HASH_MORTON, 4), (L.HASH_LINEAR, 0), (L.HASH_MORTON, 0)])
+def test_subcell_walk_matches_plain_walk_and_oracle(device, oracle, order, xbits):
+    """the sub-cell walk order against the reference-order walk and the oracle:
+    hash, index and cell ranges bit for bit, the sorted rows and the order of
+    measures unchanged, density and force equal to rounding (other summation
+    order), and the same trajectory over 10 steps (graph replay included)"""
+    sc = S.dambreak3d(dx=0.02, hash_order=order, morton_xbits=xbits)
+    a = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    b = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    assert a.subcell_walk
+    b.set_subcell_walk(False)
+    assert not b.subcell_walk
+    sim = oracle.Simulation(sc.params, sc.pos)
+    a.update()
+    b.update()
+    sim.update(1)
+    check_grid_state(a, sim)
+    for arr in (L.ARRAY_HASH, L.ARRAY_INDEX):
+        assert np.array_equal(a.getUintArray(arr), b.getUintArray(arr))
+    assert np.array_equal(a.getArray(L.ARRAY_SORTEDPOS), b.getArray(L.ARRAY_SORTEDPOS))
+    ma, mb = a.getArray(L.ARRAY_MEASURES), b.getArray(L.ARRAY_MEASURES)
+    assert np.max(np.abs(ma[:, 0] - mb[:, 0]) / mb[:, 0]) <= 1e-6
+    assert np.max(np.abs(ma[:, 0] - sim.measures[:, 0]) / sim.measures[:, 0]) <= 1e-5
+    nf = sc.num_fluid
+    acc_a, acc_b = a.getArray(L.ARRAY_ACCELERATION)[:nf], b.getArray(L.ARRAY_ACCELERATION)[:nf]
+    assert rel_field(acc_a, acc_b) <= 1e-6
+    assert rel_field(acc_a, sim.acc[:nf]) <= 1e-5
+    a.run(5, use_graph=True)
+    a.run(4, use_graph=False)
+    b.run(9, use_graph=False)
+    sim.update(9)
+    check_grid_state(a, sim)
+    assert rel_field(a.getArray(L.ARRAY_POSITION), b.getArray(L.ARRAY_POSITION)) <= 1e-6
+    assert rel_field(a.getArray(L.ARRAY_POSITION), sim.pos) <= 1e-6
+
+
+def test_subcell_walk_rejected_where_it_does_not_apply(device):
+    sc = S.poiseuille_channel()
+    s = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
+    assert not s.subcell_walk
+    with pytest.raises(RuntimeError, match="sub-cell"):
+        s.set_subcell_walk(True)
+    s.set_subcell_walk(False)
+
+
 def test_graph_replay_equals_eager(device):
     sc = S.poiseuille_channel()
     a = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
