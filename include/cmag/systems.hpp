@@ -52,6 +52,11 @@ public:
     }
     float3 getCellSize() const { return make_float3(p().cell_size[0], p().cell_size[1], p().cell_size[2]); }
 
+    // neighbour-walk order (DamBreak: sub-cell order by default; off = the
+    // reference summation order)
+    void setSubcellWalk(bool on) { check(cmag_system_set_subcell_walk(s_, on ? 1 : 0), "setSubcellWalk"); }
+    bool getSubcellWalk() const { return cmag_system_subcell_walk(s_) != 0; }
+
     // access to the C handle (streams, graphs, stage timing)
     cmag_system* handle() const { return s_; }
     cmag_params params() const { return p(); }
