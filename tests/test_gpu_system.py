@@ -148,6 +148,38 @@ def test_subcell_walk_matches_plain_walk_and_oracle(device, oracle, order, xbits
     assert rel_field(a.getArray(L.ARRAY_POSITION), sim.pos) <= 1e-6
 
 
+def test_subcell_walk_on_quarter_boundaries(device, oracle):
+    """particles exactly on the quarter boundaries of their cells (x on a
+    lattice of cell/4), fluid and wall mixed in every cell: the sub-cell walk
+    equals the reference-order walk to rounding and the oracle to 1e-5"""
+    sc = S.dambreak3d(dx=0.02, hash_order=L.HASH_MORTON, morton_xbits=4)
+    p = sc.params
+    cs = float(p.cell_size[0])
+    rng = np.random.default_rng(7)
+    n = 6000
+    x0 = float(p.world_origin[0]) + cs * np.floor((0.05 - float(p.world_origin[0])) / cs)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = x0 + (cs / 4) * rng.integers(0, 40, n)
+    pos[:, 1] = rng.uniform(0.0, 0.3, n)
+    pos[:, 2] = rng.uniform(0.0, 0.3, n)
+    pos[:, 3] = np.where(rng.uniform(size=n) < 0.2, 1.0, 4.0)  # CMAG_DB_FIRST walls, fluid
+    a = ParamSystem.from_state(p, pos, None)
+    b = ParamSystem.from_state(p, pos, None)
+    b.set_subcell_walk(False)
+    sim = oracle.Simulation(p, pos)
+    a.update()
+    b.update()
+    sim.update(1)
+    check_grid_state(a, sim)
+    ma, mb = a.getArray(L.ARRAY_MEASURES), b.getArray(L.ARRAY_MEASURES)
+    assert np.max(np.abs(ma[:, 0] - mb[:, 0]) / mb[:, 0]) <= 1e-6
+    assert np.max(np.abs(ma[:, 0] - sim.measures[:, 0]) / sim.measures[:, 0]) <= 1e-5
+    fl = pos[:, 3] == 4.0
+    acc_a, acc_b = a.getArray(L.ARRAY_ACCELERATION)[fl], b.getArray(L.ARRAY_ACCELERATION)[fl]
+    assert rel_field(acc_a, acc_b) <= 1e-6
+    assert rel_field(acc_a, sim.acc[fl]) <= 1e-5
+
+
 def test_subcell_walk_rejected_where_it_does_not_apply(device):
     sc = S.poiseuille_channel()
     s = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
