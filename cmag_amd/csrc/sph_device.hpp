@@ -382,19 +382,19 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
         return;
     }
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const float inv_h = 1.0f / P.smoothing_radius;
+    const float half_inv_h = -0.5f / P.smoothing_radius;
     float sum = 0.0f;
     walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
                                                          [&](uint32_t, const float4& pj, bool valid, float px,
                                                              float py, float pz, int) {
         // a candidate past the run sits at r^2 = 1e30: t = 0 and it adds +0
         const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
-        // (1 - q/2) clamped at 0 contributes nothing beyond 2h: no branch
-        const float q = fsqrt(r2) * inv_h;
-        // 1 - q/2 <= 1, so the clamp to [0, 1] is max(0, .) and folds into the fma
-        const float t = fminf(fmaxf(fmaf(-0.5f, q, 1.0f), 0.0f), 1.0f);
+        // t = 1 - q/2 with q = r/h, clamped at 0 (nothing beyond 2h, no
+        // branch); 1 - q/2 <= 1, so the clamp to [0, 1] is max(0, .) and folds
+        // into the fma.  (1 - q/2)^4 (2q + 1) = t^4 (5 - 4t)
+        const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
         const float t2 = t * t;
-        sum = fmaf(t2 * t2, fmaf(2.0f, q, 1.0f), sum);
+        sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
     });
     const float dens = sum * P.kernel_coeff * P.particle_mass;
     if (SYS == CMAG_SYS_PERISTALSIS) {
