@@ -59,6 +59,14 @@ __device__ __forceinline__ Cell grid_pos(const cmag_params& P, float x, float y,
     return c;
 }
 
+// quarter (0..3) of a position's cell along x, from the quotient that
+// grid_pos floors (the DIV_CELL rule); the sub-cell walk order buckets by it
+__device__ __forceinline__ int x_quarter(const cmag_params& P, float x) {
+#pragma clang fp contract(off)
+    const float u = (x - P.world_origin[0]) / P.cell_size[0];
+    return min(3, (int)((u - floorf(u)) * 4.0f));
+}
+
 // software bit deposit of the low bits of v into the set bits of mask
 __device__ __forceinline__ uint32_t deposit_bits(uint32_t v, uint32_t mask) {
     uint32_t out = 0;

@@ -63,6 +63,9 @@ int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, ui
 // Run heads of the sorted keys mark cellStart; the element before a head ends
 // the previous cell; the last element ends its cell.  Gathers pos and the
 // leapfrog velocity into sorted order.
+// BUCKET: also the sub-cell bucket of every sorted row (DamBreak: fluid
+// quarter 0..3 of the cell along x, walls 4), one byte per row
+template <bool BUCKET>
 __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cellStart,
                                                        uint32_t* __restrict__ cellEnd,
                                                        float4* __restrict__ sortedPos,
@@ -71,7 +74,8 @@ __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cel
                                                        const uint32_t* __restrict__ index,
                                                        const float4* __restrict__ oldPos,
                                                        const float4* __restrict__ oldVel,
-                                                       uint32_t n, uint32_t numCells) {
+                                                       uint32_t n, uint32_t numCells, const cmag_params P,
+                                                       uint8_t* __restrict__ bucket) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t h = hash[i];
@@ -89,20 +93,31 @@ __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cel
     if (i == n - 1 && h < numCells) cellEnd[h] = n;
     const uint32_t k = index[i];
     if (k < n) {  // a valid permutation always passes; anything else is skipped
-        sortedPos[i] = oldPos[k];
+        const float4 p = oldPos[k];
+        sortedPos[i] = p;
         sortedVel[i] = oldVel[k];
+        if (BUCKET) bucket[i] = (uint8_t)(p.w != CMAG_DB_FLUID ? 4 : x_quarter(P, p.x));
     }
 }
 
 int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,
                    const uint32_t* hash, const uint32_t* index, const float* oldPos,
-                   const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s) {
+                   const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s, const cmag_params* P,
+                   uint8_t* bucket) {
     CMAG_TRY(launch_fill_u32(cellStart, CMAG_CELL_EMPTY, numCells, s));
     if (n == 0) return CMAG_OK;
-    hipLaunchKernelGGL(reorder_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, cellStart, cellEnd,
-                       reinterpret_cast<float4*>(sortedPos), reinterpret_cast<float4*>(sortedVel),
-                       hash, index, reinterpret_cast<const float4*>(oldPos),
-                       reinterpret_cast<const float4*>(oldVel), n, numCells);
+    auto f4 = [](float* q) { return reinterpret_cast<float4*>(q); };
+    auto c4 = [](const float* q) { return reinterpret_cast<const float4*>(q); };
+    if (bucket) {
+        if (!P) return fail(CMAG_E_INVALID, "reorder: the buckets need the parameter block");
+        hipLaunchKernelGGL(reorder_kernel<true>, dim3(div_up(n, 256)), dim3(256), 0, s, cellStart, cellEnd,
+                           f4(sortedPos), f4(sortedVel), hash, index, c4(oldPos), c4(oldVel), n, numCells, *P,
+                           bucket);
+    } else {
+        hipLaunchKernelGGL(reorder_kernel<false>, dim3(div_up(n, 256)), dim3(256), 0, s, cellStart, cellEnd,
+                           f4(sortedPos), f4(sortedVel), hash, index, c4(oldPos), c4(oldVel), n, numCells,
+                           cmag_params{}, nullptr);
+    }
     return check_launch("reorder_kernel");
 }
 

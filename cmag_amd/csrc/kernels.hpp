@@ -17,9 +17,11 @@ int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, uint32_t* index,
                      uint32_t n, hipStream_t s);
+// with bucket (and P): also the sub-cell bucket byte of every sorted row
 int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, float* sortedVel,
                    const uint32_t* hash, const uint32_t* index, const float* oldPos,
-                   const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s);
+                   const float* oldVel, uint32_t n, uint32_t numCells, hipStream_t s,
+                   const cmag_params* P = nullptr, uint8_t* bucket = nullptr);
 
 // out = exclusive_scan(in) (in place allowed); ws holds scan_levels_words(len) words
 size_t scan_levels_words(size_t len);
@@ -43,13 +45,13 @@ int launch_density(const cmag_params& P, float* measures, const float* sortedPos
                    const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t n, hipStream_t s,
                    const uint32_t* cellFirst = nullptr);
 // sub-cell walk order (DamBreak): build the internal order from the reference
-// one (pre: 4 uint32 per cell, cleared here; bucket: n bytes of scratch), then
-// density and force over it; density also writes the reference-order measures
-// through w2r
+// one (pre: 4 uint32 per cell, cleared here; bucket: the rows' buckets from
+// launch_reorder, padded to whole 16-byte words + 16), then density and force
+// over it; density also writes the reference-order measures through w2r
 int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint32_t* index, const float* sortedPos,
                          const float* sortedVel, const uint32_t* cellFirst, uint32_t n, uint32_t numCells,
-                         uint32_t* pre, uint8_t* bucket, float* posW, float* velW, uint32_t* indexW, uint32_t* w2r,
-                         hipStream_t s);
+                         uint32_t* pre, const uint8_t* bucket, float* posW, float* velW, uint32_t* indexW,
+                         uint32_t* w2r, hipStream_t s);
 int launch_density_sub(const cmag_params& P, float* measuresW, float* measuresR, const uint32_t* w2r,
                        const float* posW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
                        hipStream_t s);

@@ -147,21 +147,14 @@ int launch_force(const cmag_params& P, float* acc, const float* measures, const 
 }
 
 // ---- sub-cell walk order (DamBreak) ---------------------------------------------
-// subcell_bucket_kernel: the bucket of every sorted row (one byte).
-// subcell_move_kernel: row i of the reference order (cell c = hash[i]) moves
-// to row first[c] + below of the sub-cell order, below = the rows of the cell
-// in a lower bucket or in the same bucket and earlier in reference order; the
-// cell's first row also writes the running bucket counts pre[c] (cells
-// without particles keep the zeros they were cleared to).  O(cell size) work
-// per row (~18 at the measured sizes) over 16-byte loads of the buckets.
-__global__ __launch_bounds__(256) void subcell_bucket_kernel(const cmag_params P, const float4* __restrict__ sortedPos,
-                                                             uint32_t n, uint8_t* __restrict__ bucket) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = sortedPos[i];
-    bucket[i] = (uint8_t)(p.w != CMAG_DB_FLUID ? 4 : x_quarter(P, p.x));
-}
-
+// The bucket of every sorted row (a byte: fluid quarter 0..3 of the cell
+// along x, walls 4) comes from the reorder.  subcell_move_kernel: row i of
+// the reference order (cell c = hash[i]) moves to row first[c] + below of the
+// sub-cell order, below = the rows of the cell in a lower bucket or in the
+// same bucket and earlier in reference order; the cell's first row also
+// writes the running bucket counts pre[c] (cells without particles keep the
+// zeros they were cleared to).  O(cell size) work per row (~18 at the
+// measured sizes) over 16-byte loads of the buckets.
 __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __restrict__ hash,
                                                            const uint32_t* __restrict__ index,
                                                            const float4* __restrict__ sortedPos,
@@ -205,13 +198,11 @@ __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __res
 
 int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint32_t* index, const float* sortedPos,
                          const float* sortedVel, const uint32_t* cellFirst, uint32_t n, uint32_t numCells,
-                         uint32_t* pre, uint8_t* bucket, float* posW, float* velW, uint32_t* indexW, uint32_t* w2r,
-                         hipStream_t s) {
+                         uint32_t* pre, const uint8_t* bucket, float* posW, float* velW, uint32_t* indexW,
+                         uint32_t* w2r, hipStream_t s) {
+    (void)P;
     if (n == 0) return CMAG_OK;
     CMAG_TRY(launch_fill_u32(pre, 0u, 4 * (size_t)numCells, s));
-    hipLaunchKernelGGL(subcell_bucket_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, P,
-                       reinterpret_cast<const float4*>(sortedPos), n, bucket);
-    CMAG_TRY(check_launch("subcell_bucket_kernel"));
     hipLaunchKernelGGL(subcell_move_kernel, dim3(div_up(n, 256)), dim3(256), 0, s, hash, index,
                        reinterpret_cast<const float4*>(sortedPos), reinterpret_cast<const float4*>(sortedVel),
                        cellFirst, bucket, n, reinterpret_cast<uint4*>(pre), reinterpret_cast<float4*>(posW),

@@ -109,13 +109,7 @@ struct FirstCells {
 // away, beyond 2h), and of the last cell only the quarters <= q.  Wall
 // particles sit in their own bucket and are always walked: DamBreak's
 // Lennard-Jones term acts on every wall particle of the stencil.
-// x_quarter is the target's / particle's quarter, from the quotient that
-// grid_pos floors (the DIV_CELL rule).
-__device__ __forceinline__ int x_quarter(const cmag_params& P, float x) {
-#pragma clang fp contract(off)
-    const float u = (x - P.world_origin[0]) / P.cell_size[0];
-    return min(3, (int)((u - floorf(u)) * 4.0f));
-}
+// x_quarter (common.hpp) is the target's / particle's quarter.
 
 // SubCells: cell offsets first[] as FirstCells plus, per cell, the running
 // counts pre[c] = (#bucket 0, #buckets <= 1, #buckets <= 2, #fluid); cells

@@ -177,7 +177,7 @@ struct cmag_system {
         }
         CMAG_TRY(mark(CMAG_STAGE_REORDER));
         CMAG_TRY(launch_reorder(cellStart, cellEnd, sortedPos, sortedVel, hash, index, pos, velLF, n,
-                                numCells, stream));
+                                numCells, stream, &P, subwalk ? bucketR : nullptr));
         CMAG_TRY(launch_cell_offsets(cellStart, cellEnd, cellFirst, numCells, n, scanWs, stream));
         if (subwalk)
             CMAG_TRY(launch_subcell_build(P, hash, index, sortedPos, sortedVel, cellFirst, n, numCells, preW, bucketR,
