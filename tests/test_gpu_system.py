@@ -189,8 +189,10 @@ def test_subcell_walk_rejected_where_it_does_not_apply(device):
     s.set_subcell_walk(False)
 
 
-def test_graph_replay_equals_eager(device):
-    sc = S.poiseuille_channel()
+@pytest.mark.parametrize("case", ["channel", "dam3d_subcell"])
+def test_graph_replay_equals_eager(device, case):
+    sc = S.poiseuille_channel() if case == "channel" else S.dambreak3d(dx=0.02, hash_order=L.HASH_MORTON,
+                                                                          morton_xbits=4)
     a = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
     b = ParamSystem.from_state(sc.params, sc.pos, sc.vel)
     a.run(25, use_graph=False)
