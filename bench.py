@@ -289,7 +289,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--dx", type=float, default=0.4 / 202)
     ap.add_argument("--hash", choices=["morton", "linear"], default="morton")
-    ap.add_argument("--morton-xbits", type=int, default=4,
+    ap.add_argument("--morton-xbits", type=int, default=7,
                     help="Z order over bricks of 2^xbits cells along x (0 = plain Morton)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-channel comparison")

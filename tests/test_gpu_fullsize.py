@@ -99,7 +99,7 @@ def test_dambreak_1m_500_steps_checked_every_50(device, oracle):
 
 
 def test_dambreak_16m_sort_cells_and_sampled_fields(device, oracle):
-    sc = S.dambreak3d(dx=0.4 / 202, hash_order=L.HASH_MORTON, morton_xbits=4)
+    sc = S.dambreak3d(dx=0.4 / 202, hash_order=L.HASH_MORTON, morton_xbits=7)
     p = sc.params
     print(f"16M dam break: {sc.num_fluid} fluid + {sc.meta['num_wall']} wall particles", flush=True)
     s = ParamSystem.from_state(p, sc.pos, sc.vel)
