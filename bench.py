@@ -46,6 +46,8 @@ ALG_BYTES = {
     "reorder": 4 + 4 + 16 + 16 + 32,     # hash, index, gathered pos/velLF -> sortedPos/Vel
     "density": 16 + 16,                  # sortedPos -> measures
     "force": 16 + 16 + 16 + 4 + 16,      # sortedPos/Vel, measures, index -> acc
+    # sub-cell walk: rho and p ride in the w of the row copies, no measures read
+    "force_sub": 16 + 16 + 4 + 16,
     "integrate": 64 + 48,                # pos, vel, acc, velLF -> pos, vel, velLF
 }
 
@@ -355,7 +357,8 @@ def main() -> int:
     per_stage = {}
     for name, (ms, cnt) in stages.items():
         avg_ms = ms / max(cnt, 1)
-        alg = ALG_BYTES[name] * n_total + cell_bytes(name, num_cells, occupied)
+        key = "force_sub" if name == "force" and sysobj.subcell_walk else name
+        alg = ALG_BYTES[key] * n_total + cell_bytes(name, num_cells, occupied)
         per_stage[name] = {"avg_ms": round(avg_ms, 4), "alg_bytes": alg,
                            "alg_GBs": round(alg / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
     dominant = max(per_stage, key=lambda k: per_stage[k]["avg_ms"])

@@ -47,15 +47,15 @@ int launch_density(const cmag_params& P, float* measures, const float* sortedPos
 // sub-cell walk order (DamBreak): build the internal order from the reference
 // one (pre: 4 uint32 per cell, cleared here; bucket: the rows' buckets from
 // launch_reorder, padded to whole 16-byte words + 16), then density and force
-// over it; density also writes the reference-order measures through w2r
+// over it; density writes the reference-order measures through w2r and leaves
+// -rho (fluid rows) in posW.w and p in velW.w for the force pass
 int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint32_t* index, const float* sortedPos,
                          const float* sortedVel, const uint32_t* cellFirst, uint32_t n, uint32_t numCells,
                          uint32_t* pre, const uint8_t* bucket, float* posW, float* velW, uint32_t* indexW,
                          uint32_t* w2r, hipStream_t s);
-int launch_density_sub(const cmag_params& P, float* measuresW, float* measuresR, const uint32_t* w2r,
-                       const float* posW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
-                       hipStream_t s);
-int launch_force_sub(const cmag_params& P, float* acc, const float* measuresW, const float* posW, const float* velW,
+int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2r, float* posW, float* velW,
+                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s);
+int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const float* velW,
                      const uint32_t* indexW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
                      hipStream_t s);
 int launch_force(const cmag_params& P, float* acc, const float* measures, const float* sortedPos,

@@ -210,38 +210,48 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
     return check_launch("subcell_move_kernel");
 }
 
+// Density in sub-cell order.  The result goes to the reference-order row
+// measures[w2r[i]] and, for a fluid row, into the sub-cell copies that the
+// force pass reads: -rho in posW[i].w and p in velW[i].w.  Other threads read
+// posW rows during this pass, but only their x, y, z; a thread writes the w of
+// its own row only, after its walk.
 template <int DIM, int HASH>
-__global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, float4* __restrict__ measuresW,
-                                                           float4* __restrict__ measuresR,
-                                                           const uint32_t* __restrict__ w2r,
-                                                           const float4* __restrict__ posW, const SubCells cells,
-                                                           uint32_t n) {
+__global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, float4* __restrict__ measures,
+                                                           const uint32_t* __restrict__ w2r, float4* posW,
+                                                           float4* velW, const SubCells cells, uint32_t n) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    density_body<CMAG_SYS_DAMBREAK, DIM, HASH>(P, i, measuresW, posW, cells, measuresR, w2r);
+    const float4 pi = posW[i];
+    const float dens = density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells);
+    const float p = eos(P, dens);
+    reinterpret_cast<float2*>(measures)[2 * (size_t)w2r[i]] = make_float2(dens, p);
+    if (pi.w == CMAG_DB_FLUID) {
+        reinterpret_cast<float*>(posW)[4 * (size_t)i + 3] = -dens;
+        reinterpret_cast<float*>(velW)[4 * (size_t)i + 3] = p;
+    }
 }
 
-int launch_density_sub(const cmag_params& P, float* measuresW, float* measuresR, const uint32_t* w2r,
-                       const float* posW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
-                       hipStream_t s) {
+int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2r, float* posW, float* velW,
+                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s) {
     if (n == 0) return CMAG_OK;
     if (P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_INVALID, "the sub-cell walk is DamBreak's");
     const SubCells cells{cellFirst, reinterpret_cast<const uint4*>(pre)};
-    auto f4 = [](float* q) { return reinterpret_cast<float4*>(q); };
-    const float4* pw = reinterpret_cast<const float4*>(posW);
+    float4* m4 = reinterpret_cast<float4*>(measures);
+    float4* pw = reinterpret_cast<float4*>(posW);
+    float4* vw = reinterpret_cast<float4*>(velW);
     dim3 g(div_up(n, 256)), b(256);
     const bool m = P.hash_order == CMAG_HASH_MORTON;
     if (P.dim == 3) {
-        if (m) hipLaunchKernelGGL((density_sub_kernel<3, 1>), g, b, 0, s, P, f4(measuresW), f4(measuresR), w2r, pw, cells, n);
-        else hipLaunchKernelGGL((density_sub_kernel<3, 0>), g, b, 0, s, P, f4(measuresW), f4(measuresR), w2r, pw, cells, n);
+        if (m) hipLaunchKernelGGL((density_sub_kernel<3, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
+        else hipLaunchKernelGGL((density_sub_kernel<3, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
     } else {
-        if (m) hipLaunchKernelGGL((density_sub_kernel<2, 1>), g, b, 0, s, P, f4(measuresW), f4(measuresR), w2r, pw, cells, n);
-        else hipLaunchKernelGGL((density_sub_kernel<2, 0>), g, b, 0, s, P, f4(measuresW), f4(measuresR), w2r, pw, cells, n);
+        if (m) hipLaunchKernelGGL((density_sub_kernel<2, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
+        else hipLaunchKernelGGL((density_sub_kernel<2, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
     }
     return check_launch("density_sub_kernel");
 }
 
-int launch_force_sub(const cmag_params& P, float* acc, const float* measuresW, const float* posW, const float* velW,
+int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const float* velW,
                      const uint32_t* indexW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
                      hipStream_t s) {
     if (n == 0) return CMAG_OK;
@@ -251,7 +261,7 @@ int launch_force_sub(const cmag_params& P, float* acc, const float* measuresW, c
     auto run = [&](auto dim, auto hash) {
         hipLaunchKernelGGL((force_kernel<CMAG_SYS_DAMBREAK, decltype(dim)::value, decltype(hash)::value, SubCells>),
                            dim3(div_up(n, 256)), dim3(256), 0, s, P, reinterpret_cast<float4*>(acc),
-                           reinterpret_cast<const float4*>(measuresW), reinterpret_cast<const float4*>(posW),
+                           nullptr, reinterpret_cast<const float4*>(posW),
                            reinterpret_cast<const float4*>(velW), indexW, cells, n);
     };
     const bool m = P.hash_order == CMAG_HASH_MORTON;

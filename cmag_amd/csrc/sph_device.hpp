@@ -362,19 +362,11 @@ struct NbQueue {
 };
 
 // ---------------------------------------------------------------------------
-// measuresR / w2r (optional): also write the result to measuresR[w2r[i]],
-// the reference-order row of sorted row i when the walk runs in sub-cell order
+// the density of the particle at pi: the mass-weighted kernel sum over the
+// stencil (the particle itself included)
 template <int SYS, int DIM, int HASH, typename Cells>
-__device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, float4* __restrict__ measures,
-                                             const float4* __restrict__ sortedPos, const Cells& cells,
-                                             float4* __restrict__ measuresR = nullptr,
-                                             const uint32_t* __restrict__ w2r = nullptr) {
-    const float4 pi = sortedPos[i];
-    float2* m2 = reinterpret_cast<float2*>(measures);
-    if (SYS == CMAG_SYS_POISEUILLE && pi.w == CMAG_PO_WALL) {
-        m2[2 * (size_t)i] = make_float2(P.rest_density, eos(P, P.rest_density));
-        return;
-    }
+__device__ __forceinline__ float density_sum(const cmag_params& P, const float4& pi,
+                                             const float4* __restrict__ sortedPos, const Cells& cells) {
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     float sum = 0.0f;
@@ -390,13 +382,23 @@ __device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, f
         const float t2 = t * t;
         sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
     });
-    const float dens = sum * P.kernel_coeff * P.particle_mass;
+    return sum * P.kernel_coeff * P.particle_mass;
+}
+
+template <int SYS, int DIM, int HASH, typename Cells>
+__device__ __forceinline__ void density_body(const cmag_params& P, uint32_t i, float4* __restrict__ measures,
+                                             const float4* __restrict__ sortedPos, const Cells& cells) {
+    const float4 pi = sortedPos[i];
+    float2* m2 = reinterpret_cast<float2*>(measures);
+    if (SYS == CMAG_SYS_POISEUILLE && pi.w == CMAG_PO_WALL) {
+        m2[2 * (size_t)i] = make_float2(P.rest_density, eos(P, P.rest_density));
+        return;
+    }
+    const float dens = density_sum<SYS, DIM, HASH>(P, pi, sortedPos, cells);
     if (SYS == CMAG_SYS_PERISTALSIS) {
         measures[i] = make_float4(dens, eos(P, dens), 0.0f, pi.w);
     } else {
-        const float2 m = make_float2(dens, eos(P, dens));
-        m2[2 * (size_t)i] = m;
-        if (measuresR) reinterpret_cast<float2*>(measuresR)[2 * (size_t)w2r[i]] = m;
+        m2[2 * (size_t)i] = make_float2(dens, eos(P, dens));
     }
 }
 
@@ -507,6 +509,10 @@ struct ForceSum {
 // depth); pair terms take entries two at a time with all six row loads issued
 // before either term.
 // nbl: this wave's queues in LDS, [CAP][64]
+// In the sub-cell walk (Cells::SUB, DamBreak) the row copies carry the
+// density pass's results: a fluid row holds -rho in pos.w, sign bit set (a wall
+// row keeps its type flag, 0..3, sign bit clear) and p in vel.w, so a pair term gathers two rows instead of
+// three and measures is not read.
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -515,16 +521,21 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                                            const uint32_t* __restrict__ index, const Cells& cells, uint32_t n,
                                            uint32_t (*nbl)[64]) {
     const int lane = threadIdx.x & 63;
+    constexpr bool PACKED = SYS == CMAG_SYS_DAMBREAK && Cells::SUB;
+    auto is_wall = [](float w) { return PACKED ? __float_as_int(w) >= 0 : w != CMAG_DB_FLUID; };
     const float4 pi = sortedPos[i];
-    if (SYS == CMAG_SYS_DAMBREAK && pi.w != CMAG_DB_FLUID) return;
-    ForceSum<SYS> F(P, pi, sortedVel[i], measures[i]);
+    if (SYS == CMAG_SYS_DAMBREAK && is_wall(pi.w)) return;
+    const float4 vi = sortedVel[i];
+    ForceSum<SYS> F(P, pi, vi, PACKED ? make_float4(-pi.w, vi.w, 0.0f, 0.0f) : measures[i]);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
     NbQueue<CAP> nq(nbl, lane);
     auto pairs = [&](uint32_t e0, uint32_t e1, bool second) {
         const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
-        const float4 p0 = row4(sortedPos, j0), m0 = row4(measures, j0), v0 = row4(sortedVel, j0);
-        const float4 p1 = row4(sortedPos, j1), m1 = row4(measures, j1), v1 = row4(sortedVel, j1);
+        const float4 p0 = row4(sortedPos, j0), v0 = row4(sortedVel, j0);
+        const float4 p1 = row4(sortedPos, j1), v1 = row4(sortedVel, j1);
+        const float4 m0 = PACKED ? make_float4(-p0.w, v0.w, 0.0f, 0.0f) : row4(measures, j0);
+        const float4 m1 = PACKED ? make_float4(-p1.w, v1.w, 0.0f, 0.0f) : row4(measures, j1);
         F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e0)), p0, m0, v0);
         F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e1)), p1, m1, v1, second);
     };
@@ -535,7 +546,7 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
         const float r2 = norm2(rx, ry, rz);
         if (valid && j != i) {
-            if (SYS == CMAG_SYS_DAMBREAK && pj.w != CMAG_DB_FLUID) {
+            if (SYS == CMAG_SYS_DAMBREAK && is_wall(pj.w)) {
                 // Lennard-Jones wall term: every wall particle of the stencil acts
                 F.wall(rx, ry, rz, r2);
             } else if (r2 < r2max) {

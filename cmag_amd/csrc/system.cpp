@@ -37,10 +37,11 @@ struct cmag_system {
     // adjoint scratch (sorted order), allocated by the first vjp_step
     float *gaccU = nullptr, *gaccS = nullptr, *gposS = nullptr, *gvelS = nullptr, *grho = nullptr;
     // sub-cell walk order (DamBreak): the neighbour passes run over copies of
-    // the sorted rows in bucket order inside each cell; measures also land in
-    // the reference order, so every array a caller sees keeps its meaning
+    // the sorted rows in bucket order inside each cell; the density pass writes
+    // measures in the reference order, so every array a caller sees keeps its
+    // meaning, and -rho / p into the copies' w for the force pass
     bool subwalk = false;
-    float *posW = nullptr, *velW = nullptr, *measW = nullptr;
+    float *posW = nullptr, *velW = nullptr;
     uint32_t *indexW = nullptr, *w2r = nullptr, *preW = nullptr;
     uint8_t* bucketR = nullptr;
 
@@ -88,7 +89,7 @@ struct cmag_system {
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
-                        gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, measW, indexW, w2r, preW, bucketR};
+                        gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, indexW, w2r, preW, bucketR};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -143,7 +144,7 @@ struct cmag_system {
         if (on && !posW) {
             const size_t f4 = sizeof(float) * 4 * (size_t)(n ? n : 1);
             const size_t u = sizeof(uint32_t) * (size_t)(n ? n : 1);
-            for (float** b : {&posW, &velW, &measW}) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
+            for (float** b : {&posW, &velW}) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
             for (uint32_t** b : {&indexW, &w2r}) CMAG_HIP_TRY(hipMalloc((void**)b, u));
             // padded to whole 16-byte words: the move kernel reads buckets 16 at a time
             CMAG_HIP_TRY(hipMalloc((void**)&bucketR, ((size_t)n + 31) & ~(size_t)15));
@@ -184,7 +185,7 @@ struct cmag_system {
                                           posW, velW, indexW, w2r, stream));
         CMAG_TRY(mark(CMAG_STAGE_DENSITY));
         if (subwalk)
-            CMAG_TRY(launch_density_sub(P, measW, measures, w2r, posW, cellFirst, preW, n, stream));
+            CMAG_TRY(launch_density_sub(P, measures, w2r, posW, velW, cellFirst, preW, n, stream));
         else
             CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
         return CMAG_OK;
@@ -198,7 +199,7 @@ struct cmag_system {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
                                                cellStart, cellEnd, n, stream, cellFirst));
         } else if (subwalk) {
-            CMAG_TRY(launch_force_sub(P, acc, measW, posW, velW, indexW, cellFirst, preW, n, stream));
+            CMAG_TRY(launch_force_sub(P, acc, posW, velW, indexW, cellFirst, preW, n, stream));
         } else {
             CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
                                   stream, cellFirst));
