@@ -48,7 +48,7 @@ int launch_density(const cmag_params& P, float* measures, const float* sortedPos
 // one (pre: 4 uint32 per cell, cleared here; bucket: the rows' buckets from
 // launch_reorder, padded to whole 16-byte words + 16), then density and force
 // over it; density writes the reference-order measures through w2r and leaves
-// -rho (fluid rows) in posW.w and p in velW.w for the force pass
+// -rho (fluid rows) in posW.w and p / rho^2 in velW.w for the force pass
 int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint32_t* index, const float* sortedPos,
                          const float* sortedVel, const uint32_t* cellFirst, uint32_t n, uint32_t numCells,
                          uint32_t* pre, const uint8_t* bucket, float* posW, float* velW, uint32_t* indexW,

@@ -212,7 +212,7 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
 
 // Density in sub-cell order.  The result goes to the reference-order row
 // measures[w2r[i]] and, for a fluid row, into the sub-cell copies that the
-// force pass reads: -rho in posW[i].w and p in velW[i].w.  Other threads read
+// force pass reads: -rho in posW[i].w and p / rho^2 in velW[i].w.  Other threads read
 // posW rows during this pass, but only their x, y, z; a thread writes the w of
 // its own row only, after its walk.
 template <int DIM, int HASH>
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, f
     reinterpret_cast<float2*>(measures)[2 * (size_t)w2r[i]] = make_float2(dens, p);
     if (pi.w == CMAG_DB_FLUID) {
         reinterpret_cast<float*>(posW)[4 * (size_t)i + 3] = -dens;
-        reinterpret_cast<float*>(velW)[4 * (size_t)i + 3] = p;
+        reinterpret_cast<float*>(velW)[4 * (size_t)i + 3] = p / (dens * dens);
     }
 }
 

@@ -440,14 +440,16 @@ template <int SYS>
 struct ForceSum {
     const cmag_params& P;
     float4 pi, vi;
-    float dens, inv_dens, pterm_i, inv_h, m, gc, av_k, av_eps, visc_k;
+    float dens, inv_dens, pterm_i, inv_h, m, gc, av_k, av_eps, visc_k, t_k, w_k;
     float fx = 0.f, fy = 0.f, fz = 0.f, wx = 0.f, wy = 0.f, wz = 0.f;
 
-    __device__ __forceinline__ ForceSum(const cmag_params& P_, const float4& pi_, const float4& vi_, const float4& mi)
+    // rho, pterm: the target's density and p / rho^2
+    __device__ __forceinline__ ForceSum(const cmag_params& P_, const float4& pi_, const float4& vi_, float rho,
+                                        float pterm)
         : P(P_), pi(pi_), vi(vi_) {
-        dens = mi.x;
+        dens = rho;
         inv_dens = 1.0f / dens;
-        pterm_i = mi.y / (dens * dens);
+        pterm_i = pterm;
         const float h = P.smoothing_radius;
         inv_h = 1.0f / h;
         m = P.particle_mass;
@@ -455,6 +457,8 @@ struct ForceSum {
         av_k = 2.0f * P.alpha * h * P.soundspeed;  // nu * (rho_i + rho_j)
         av_eps = 0.001f * h * h;
         visc_k = m * (P.mu + P.mu);
+        t_k = -0.5f * inv_h;
+        w_k = -5.0f * m * gc;
     }
 
     // Lennard-Jones wall term (DamBreak), fluid_kernel.cu:199-208
@@ -467,10 +471,33 @@ struct ForceSum {
         wx = fmaf(sc, rx, wx); wy = fmaf(sc, ry, wy); wz = fmaf(sc, rz, wz);
     }
 
-    // pressure + artificial viscosity (DamBreak) or + Morris viscosity
-    // (Poiseuille) of an in-range neighbour; one reciprocal square root and
-    // one reciprocal per pair (two with artificial viscosity); valid = false
-    // adds nothing
+    // DamBreak pressure + artificial viscosity of an in-range fluid neighbour
+    // (fluid_kernel.cu:210-262) with density rho_j and pterm_j = p_j / rho_j^2.
+    // With t = 1 - q/2 the kernel gradient (1-q/2)^4 - (1-q/2)^3 (2q+1) is
+    // 5 t^3 (t - 1); the -m gc factors fold into one constant.  The viscosity
+    // term is branch-free (min(v.r, 0) makes it +0 for separating pairs), which
+    // also keeps rho_j's load with the rest of its row.  One reciprocal square
+    // root and one reciprocal per pair.  valid = false adds +0: the term is
+    // scaled by 0 rather than skipped (a skipped term would move its row loads
+    // into a branch); an invalid entry repeats a valid one, so its values are
+    // finite.
+    __device__ __forceinline__ void pair_db(const float3& q, const float4& pj, const float4& vj, float rho_j,
+                                            float pterm_j, bool valid = true) {
+        const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
+        const float r2 = norm2(rx, ry, rz);
+        const float inv_d = frsq(r2);
+        const float t = fmaf(t_k, r2 * inv_d, 1.0f);
+        const float t3 = t * t * t;
+        const float vr = fmaf(vi.z - vj.z, rz, fmaf(vi.y - vj.y, ry, (vi.x - vj.x) * rx));
+        const float av = -av_k * fminf(vr, 0.0f) * frcp((dens + rho_j) * (r2 + av_eps));
+        const float s = (pterm_i + pterm_j) + av;
+        const float kk = s * inv_d * t3 * fmaf(w_k, t, -w_k) * (valid ? 1.0f : 0.0f);
+        fx = fmaf(kk, rx, fx); fy = fmaf(kk, ry, fy); fz = fmaf(kk, rz, fz);
+    }
+
+    // Poiseuille pressure + Morris viscosity of an in-range neighbour
+    // (poiseuilleFlowKernel.cu:184-270); one reciprocal square root and one
+    // reciprocal per pair; valid = false adds nothing
     __device__ __forceinline__ void pair(const float3& q, const float4& pj, const float4& mj, const float4& vj,
                                          bool valid = true) {
         const float rx = q.x - pj.x, ry = q.y - pj.y, rz = q.z - pj.z;
@@ -482,20 +509,13 @@ struct ForceSum {
         const float t2 = t * t;
         const float temp = gc * fmaf(t2, t2, -(t2 * t) * fmaf(2.0f, qq, 1.0f));
         const float inv_dj = frcp(mj.x);
-        float s = fmaf(mj.y * inv_dj, inv_dj, pterm_i);
-        if (SYS == CMAG_SYS_DAMBREAK) {
-            const float vr = fmaf(vi.z - vj.z, rz, fmaf(vi.y - vj.y, ry, (vi.x - vj.x) * rx));
-            if (vr < 0.0f) s += -av_k * vr * frcp((dens + mj.x) * (r2 + av_eps));
-            const float kk = valid ? -m * s * inv_d * temp : 0.0f;
-            fx = fmaf(kk, rx, fx); fy = fmaf(kk, ry, fy); fz = fmaf(kk, rz, fz);
-        } else {
-            const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
-            const float kk = valid ? -m * s * inv_d * temp : 0.0f;
-            const float kv = valid ? visc_k * inv_dens * inv_dj * inv_d * temp : 0.0f;
-            fx += fmaf(kk, rx, kv * V.x);
-            fy += fmaf(kk, ry, kv * V.y);
-            fz += fmaf(kk, rz, kv * V.z);
-        }
+        const float s = fmaf(mj.y * inv_dj, inv_dj, pterm_i);
+        const float3 V = vel_diff_poiseuille(P, vi, pi, vj, pj);
+        const float kk = valid ? -m * s * inv_d * temp : 0.0f;
+        const float kv = valid ? visc_k * inv_dens * inv_dj * inv_d * temp : 0.0f;
+        fx += fmaf(kk, rx, kv * V.x);
+        fy += fmaf(kk, ry, kv * V.y);
+        fz += fmaf(kk, rz, kv * V.z);
     }
 };
 
@@ -511,8 +531,8 @@ struct ForceSum {
 // nbl: this wave's queues in LDS, [CAP][64]
 // In the sub-cell walk (Cells::SUB, DamBreak) the row copies carry the
 // density pass's results: a fluid row holds -rho in pos.w, sign bit set (a wall
-// row keeps its type flag, 0..3, sign bit clear) and p in vel.w, so a pair term gathers two rows instead of
-// three and measures is not read.
+// row keeps its type flag, 0..3, sign bit clear) and p / rho^2 in vel.w, so a
+// pair term gathers two rows instead of three and measures is not read.
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -526,7 +546,16 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     const float4 pi = sortedPos[i];
     if (SYS == CMAG_SYS_DAMBREAK && is_wall(pi.w)) return;
     const float4 vi = sortedVel[i];
-    ForceSum<SYS> F(P, pi, vi, PACKED ? make_float4(-pi.w, vi.w, 0.0f, 0.0f) : measures[i]);
+    float rho_i, pterm_i;
+    if (PACKED) {
+        rho_i = -pi.w;
+        pterm_i = vi.w;
+    } else {
+        const float4 mi = measures[i];
+        rho_i = mi.x;
+        pterm_i = mi.y / (mi.x * mi.x);
+    }
+    ForceSum<SYS> F(P, pi, vi, rho_i, pterm_i);
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
     NbQueue<CAP> nq(nbl, lane);
@@ -534,10 +563,22 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         const uint32_t j0 = NbEntry::index(e0), j1 = NbEntry::index(e1);
         const float4 p0 = row4(sortedPos, j0), v0 = row4(sortedVel, j0);
         const float4 p1 = row4(sortedPos, j1), v1 = row4(sortedVel, j1);
-        const float4 m0 = PACKED ? make_float4(-p0.w, v0.w, 0.0f, 0.0f) : row4(measures, j0);
-        const float4 m1 = PACKED ? make_float4(-p1.w, v1.w, 0.0f, 0.0f) : row4(measures, j1);
-        F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e0)), p0, m0, v0);
-        F.pair(image_target<SYS != CMAG_SYS_DAMBREAK>(P, pi, NbEntry::code(e1)), p1, m1, v1, second);
+        if (SYS == CMAG_SYS_DAMBREAK) {
+            const float3 q = make_float3(pi.x, pi.y, pi.z);
+            if (PACKED) {
+                F.pair_db(q, p0, v0, -p0.w, v0.w);
+                F.pair_db(q, p1, v1, -p1.w, v1.w, second);
+            } else {
+                const float4 m0 = row4(measures, j0), m1 = row4(measures, j1);
+                const float i0 = frcp(m0.x), i1 = frcp(m1.x);
+                F.pair_db(q, p0, v0, m0.x, m0.y * i0 * i0);
+                F.pair_db(q, p1, v1, m1.x, m1.y * i1 * i1, second);
+            }
+        } else {
+            const float4 m0 = row4(measures, j0), m1 = row4(measures, j1);
+            F.pair(image_target(P, pi, NbEntry::code(e0)), p0, m0, v0);
+            F.pair(image_target(P, pi, NbEntry::code(e1)), p1, m1, v1, second);
+        }
     };
 
     walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
