@@ -17,7 +17,8 @@
 
 namespace cmag {
 
-constexpr int RS_NT = 256;
+constexpr int RS_NT = 256;  // one thread per digit in the per-tile tables
+static_assert(RS_NT == 256, "the histogram and scatter tables map thread t to digit t");
 // keys per thread: 16 (4096-key tiles) for large sorts; 4 (1024-key tiles) up
 // to RS_SMALL_N keys, where a pass is a handful of tiles and the in-tile
 // ranking rounds, not bandwidth, set its duration
@@ -144,105 +145,121 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* 
 }
 
 // ---- radix passes ----------------------------------------------------------
+// per-tile digit counts: each wave counts into its own LDS histogram with
+// plain LDS atomics (lanes that hit one bin are serialised by the hardware),
+// and the block adds the wave histograms
 template <int ITEMS>
 __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
                                                          int shift, uint32_t* __restrict__ counts,
                                                          uint32_t numTiles) {
-    __shared__ uint32_t hist[256];
-    const int tid = threadIdx.x, lane = __lane_id();
-    hist[tid] = 0;
+    constexpr int NW = RS_NT / 64;
+    __shared__ uint32_t hist[NW][256];
+    const int tid = threadIdx.x, w = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) hist[k][tid] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * (uint32_t)(RS_NT * ITEMS);
     uint32_t d[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t idx = base + k * RS_NT + tid;
-        d[k] = idx < n ? (keys[idx] >> shift) & 255u : 0u;
+        const uint32_t idx = base + k * RS_NT + tid;
+        d[k] = idx < n ? (keys[idx] >> shift) & 255u : 256u;
     }
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        uint32_t idx = base + k * RS_NT + tid;
-        bool valid = idx < n;
-        uint64_t m = match_digit(d[k], valid);
-        if (valid && (__ffsll((unsigned long long)m) - 1) == lane)
-            atomicAdd(&hist[d[k]], (uint32_t)__popcll(m));
-    }
+    for (int k = 0; k < ITEMS; ++k)
+        if (d[k] < 256u) atomicAdd(&hist[w][d[k]], 1u);
     __syncthreads();
-    counts[(size_t)tid * numTiles + blockIdx.x] = hist[tid];
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) c += hist[k][tid];
+    counts[(size_t)tid * numTiles + blockIdx.x] = c;
 }
 
+// Stable scatter of one tile: wave w ranks its own contiguous chunk of 64 *
+// ITEMS keys in rounds of 64 (a key's rank is the number of keys of its digit
+// before it in the chunk: earlier rounds, kept in the wave's LDS digit counts,
+// plus lower lanes of its round), the waves' counts are turned into per-wave
+// digit offsets, and the tile is reordered by digit in LDS and written out
+// run by run.  Wave chunks, rounds and lanes follow the input order, so equal
+// digits keep their order.
 template <int ITEMS>
 __global__ __launch_bounds__(RS_NT) void rs_scatter_kernel(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int shift,
     const uint32_t* __restrict__ counts, const uint32_t* __restrict__ offsets, uint32_t numTiles) {
+    constexpr int NW = RS_NT / 64;
     constexpr uint32_t TILE = RS_NT * ITEMS;
+    constexpr uint32_t CHUNK = 64 * ITEMS;
     __shared__ uint32_t s_keys[TILE];
     __shared__ uint32_t s_vals[TILE];
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_start[256];
-    __shared__ uint32_t s_run[256];
-    __shared__ uint32_t s_wcnt[RS_NT / 64][256];
-    __shared__ uint32_t s_scan[RS_NT / 64];
+    __shared__ uint32_t s_wcnt[NW][256];
+    __shared__ uint32_t s_scan[NW];
 
     const int tid = threadIdx.x, lane = __lane_id(), w = tid / 64;
     const uint32_t tile = blockIdx.x;
     const uint32_t base = tile * TILE;
     const uint32_t cnt = min(TILE, n - base);
+    const uint32_t wbase = base + (uint32_t)w * CHUNK;
 
     uint32_t rk[ITEMS], rv[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t idx = base + k * RS_NT + tid;
+        const uint32_t idx = wbase + k * 64 + lane;
         rk[k] = idx < n ? keys_in[idx] : 0u;
         rv[k] = idx < n ? vals_in[idx] : 0u;
     }
-
-    uint32_t c = counts[(size_t)tid * numTiles + tile];
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) s_wcnt[ww][tid] = 0;
+    const uint32_t c = counts[(size_t)tid * numTiles + tile];
     s_base[tid] = offsets[(size_t)tid * numTiles + tile];
     uint32_t tot;
     s_start[tid] = block_exclusive_scan<RS_NT>(c, s_scan, &tot);
-    s_run[tid] = 0;
+    __syncthreads();
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    uint32_t rank[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
+        const bool valid = wbase + k * 64 + lane < n;
+        const uint32_t dg = (rk[k] >> shift) & 255u;
+        const uint64_t m = match_digit(dg, valid);
+        // LDS operations of one wave take effect in program order: every lane
+        // reads its digit's count before the round's leaders advance it
+        const uint32_t before = s_wcnt[w][dg];
+        rank[k] = before + (uint32_t)__popcll(m & lt_mask);
+        if (valid && (__ffsll((unsigned long long)m) - 1) == lane) s_wcnt[w][dg] = before + (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
 #pragma unroll
-        for (int ww = 0; ww < RS_NT / 64; ++ww) s_wcnt[ww][tid] = 0;
-        __syncthreads();
-        uint32_t idx = base + k * RS_NT + tid;
-        bool valid = idx < n;
-        uint32_t dg = (rk[k] >> shift) & 255u;
-        uint64_t m = match_digit(dg, valid);
-        uint32_t rank = (uint32_t)__popcll(m & lt_mask);
-        if (valid && (__ffsll((unsigned long long)m) - 1) == lane) s_wcnt[w][dg] = (uint32_t)__popcll(m);
-        __syncthreads();
-        {
-            uint32_t run = s_run[tid];
-#pragma unroll
-            for (int ww = 0; ww < RS_NT / 64; ++ww) {
-                uint32_t x = s_wcnt[ww][tid];
-                s_wcnt[ww][tid] = run;
-                run += x;
-            }
-            s_run[tid] = run;
+        for (int ww = 0; ww < NW; ++ww) {
+            const uint32_t x = s_wcnt[ww][tid];
+            s_wcnt[ww][tid] = run;
+            run += x;
         }
-        __syncthreads();
-        if (valid) {
-            uint32_t p = s_start[dg] + s_wcnt[w][dg] + rank;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (wbase + k * 64 + lane < n) {
+            const uint32_t dg = (rk[k] >> shift) & 255u;
+            const uint32_t p = s_start[dg] + s_wcnt[w][dg] + rank[k];
             s_keys[p] = rk[k];
             s_vals[p] = rv[k];
         }
-        __syncthreads();
     }
+    __syncthreads();
 
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        uint32_t p = k * RS_NT + tid;
+        const uint32_t p = k * RS_NT + tid;
         if (p < cnt) {
-            uint32_t key = s_keys[p];
-            uint32_t dg = (key >> shift) & 255u;
-            uint32_t dst = s_base[dg] + (p - s_start[dg]);
+            const uint32_t key = s_keys[p];
+            const uint32_t dg = (key >> shift) & 255u;
+            const uint32_t dst = s_base[dg] + (p - s_start[dg]);
             keys_out[dst] = key;
             vals_out[dst] = s_vals[p];
         }
