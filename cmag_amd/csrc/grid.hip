@@ -121,15 +121,6 @@ int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, flo
     return check_launch("reorder_kernel");
 }
 
-__global__ __launch_bounds__(256) void cell_count_kernel(const uint32_t* __restrict__ cellStart,
-                                                         const uint32_t* __restrict__ cellEnd,
-                                                         uint32_t* __restrict__ cnt, uint32_t numCells) {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= numCells) return;
-    const uint32_t st = cellStart[c];
-    cnt[c] = st == CMAG_CELL_EMPTY ? 0u : cellEnd[c] - st;
-}
-
 __global__ void set_word_kernel(uint32_t* p, uint32_t v) { *p = v; }
 
 // compact cell offsets for the neighbour walkers: a run of cells with
@@ -137,10 +128,7 @@ __global__ void set_word_kernel(uint32_t* p, uint32_t v) { *p = v; }
 // any empty-cell test
 int launch_cell_offsets(const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t* first,
                         uint32_t numCells, uint32_t n, uint32_t* ws, hipStream_t s) {
-    hipLaunchKernelGGL(cell_count_kernel, dim3(div_up(numCells, 256)), dim3(256), 0, s, cellStart, cellEnd,
-                       first, numCells);
-    CMAG_TRY(check_launch("cell_count_kernel"));
-    CMAG_TRY(exclusive_scan_u32(first, first, numCells, ws, s));
+    CMAG_TRY(exclusive_scan_cell_counts(cellStart, cellEnd, first, numCells, ws, s));
     hipLaunchKernelGGL(set_word_kernel, dim3(1), dim3(1), 0, s, first + numCells, n);
     return check_launch("set_word_kernel");
 }

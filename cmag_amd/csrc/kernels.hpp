@@ -26,6 +26,10 @@ int launch_reorder(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos, flo
 // out = exclusive_scan(in) (in place allowed); ws holds scan_levels_words(len) words
 size_t scan_levels_words(size_t len);
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws, hipStream_t s);
+// out = exclusive_scan of the cell counts (cellEnd - cellStart, 0 for an
+// empty cell) without materialising them; ws as above
+int exclusive_scan_cell_counts(const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t* first,
+                               size_t numCells, uint32_t* ws, hipStream_t s);
 
 // first[c] = number of particles in cells < c, first[C] = n (from the cell
 // ranges); ws holds scan_levels_words(C) words

@@ -72,8 +72,34 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool valid) {
 }
 
 // ---- exclusive scan of a uint32 array -------------------------------------
-__global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const uint32_t* __restrict__ in,
-                                                               size_t len,
+// The input is a plain array (ScanArray) or the cell counts end - start (0 for
+// an empty cell, whose end is not read for its value) computed from the
+// reorder's cell ranges as they are read (ScanCellCounts).
+struct ScanArray {
+    const uint32_t* in;
+    __device__ __forceinline__ uint32_t at(size_t i) const { return in[i]; }
+    __device__ __forceinline__ uint4 at4(size_t i) const { return *reinterpret_cast<const uint4*>(in + i); }
+    bool aligned16() const { return (reinterpret_cast<uintptr_t>(in) & 15u) == 0; }
+};
+
+struct ScanCellCounts {
+    const uint32_t* start;
+    const uint32_t* end;
+    __device__ static __forceinline__ uint32_t count(uint32_t st, uint32_t en) {
+        return st == CMAG_CELL_EMPTY ? 0u : en - st;
+    }
+    __device__ __forceinline__ uint32_t at(size_t i) const { return count(start[i], end[i]); }
+    __device__ __forceinline__ uint4 at4(size_t i) const {
+        const uint4 a = *reinterpret_cast<const uint4*>(start + i), b = *reinterpret_cast<const uint4*>(end + i);
+        return make_uint4(count(a.x, b.x), count(a.y, b.y), count(a.z, b.z), count(a.w, b.w));
+    }
+    bool aligned16() const {
+        return ((reinterpret_cast<uintptr_t>(start) | reinterpret_cast<uintptr_t>(end)) & 15u) == 0;
+    }
+};
+
+template <typename Src>
+__global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const Src src, size_t len,
                                                                uint32_t* __restrict__ partial) {
     __shared__ uint32_t lds[SCAN_NT / 64];
     size_t base = (size_t)blockIdx.x * SCAN_TILE;
@@ -81,36 +107,56 @@ __global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const uint32_t* __
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
         size_t i = base + (size_t)k * SCAN_NT + threadIdx.x;
-        sum += (i < len) ? in[i] : 0u;
+        sum += (i < len) ? src.at(i) : 0u;
     }
     uint32_t tot;
     block_exclusive_scan<SCAN_NT>(sum, lds, &tot);
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// each thread owns SCAN_ITEMS consecutive elements; `carry` (may be null)
-// holds the exclusive base of every block.  in == out is allowed: every thread
+// each thread owns SCAN_ITEMS consecutive elements, read and written as
+// 16-byte words when vec (all pointers 16-byte aligned) and the thread's
+// elements lie inside len; `carry` (may be null) holds the exclusive base of
+// every block.  An in-place scan of a plain array is allowed: every thread
 // reads its own elements before the block barrier and writes them after it.
-__global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const uint32_t* in, uint32_t* out, size_t len,
-                                                             const uint32_t* __restrict__ carry) {
+template <typename Src>
+__global__ __launch_bounds__(SCAN_NT) void scan_down_kernel(const Src src, uint32_t* out, size_t len,
+                                                             const uint32_t* __restrict__ carry, bool vec) {
+    static_assert(SCAN_ITEMS % 4 == 0, "16-byte words per thread");
     __shared__ uint32_t lds[SCAN_NT / 64];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    const bool whole = vec && base + SCAN_ITEMS <= len;
     uint32_t v[SCAN_ITEMS];
+    if (whole) {
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; ++q) {
+            const uint4 x = src.at4(base + 4 * q);
+            v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) v[k] = (base + k < len) ? src.at(base + k) : 0u;
+    }
     uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        size_t i = base + k;
-        v[k] = (i < len) ? in[i] : 0u;
-        sum += v[k];
-    }
+    for (int k = 0; k < SCAN_ITEMS; ++k) sum += v[k];
     uint32_t tot;
     uint32_t run = block_exclusive_scan<SCAN_NT>(sum, lds, &tot);
     if (carry) run += carry[blockIdx.x];
+    uint32_t o[SCAN_ITEMS];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; ++k) {
-        size_t i = base + k;
-        if (i < len) out[i] = run;
+        o[k] = run;
         run += v[k];
+    }
+    if (whole) {
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS / 4; ++q)
+            *reinterpret_cast<uint4*>(out + base + 4 * q) = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k)
+            if (base + k < len) out[base + k] = o[k];
     }
 }
 
@@ -123,25 +169,35 @@ size_t scan_levels_words(size_t len) {
     return words;
 }
 
-// out = exclusive_scan(in); workspace holds scan_levels_words(len) words
-int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws,
-                              hipStream_t s) {
+template <typename Src>
+static int exclusive_scan(const Src& src, uint32_t* out, size_t len, uint32_t* ws, hipStream_t s) {
     if (len == 0) return CMAG_OK;
+    const bool vec = src.aligned16() && (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
     if (len <= (size_t)SCAN_TILE) {
-        hipLaunchKernelGGL(scan_down_kernel, dim3(1), dim3(SCAN_NT), 0, s, in, out, len,
-                           (const uint32_t*)nullptr);
+        hipLaunchKernelGGL(scan_down_kernel<Src>, dim3(1), dim3(SCAN_NT), 0, s, src, out, len,
+                           (const uint32_t*)nullptr, vec);
         return check_launch("scan_down_kernel");
     }
     size_t blocks = (len + SCAN_TILE - 1) / SCAN_TILE;
     uint32_t* partial = ws;
-    hipLaunchKernelGGL(scan_reduce_kernel, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, in, len,
-                       partial);
+    hipLaunchKernelGGL(scan_reduce_kernel<Src>, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, src, len, partial);
     CMAG_TRY(check_launch("scan_reduce_kernel"));
     // scan the partials in place (recursively for very long inputs)
-    CMAG_TRY(exclusive_scan_u32(partial, partial, blocks, ws + blocks, s));
-    hipLaunchKernelGGL(scan_down_kernel, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, in, out, len,
-                       (const uint32_t*)partial);
+    CMAG_TRY(exclusive_scan(ScanArray{partial}, partial, blocks, ws + blocks, s));
+    hipLaunchKernelGGL(scan_down_kernel<Src>, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, src, out, len,
+                       (const uint32_t*)partial, vec);
     return check_launch("scan_down_kernel");
+}
+
+// out = exclusive_scan(in); workspace holds scan_levels_words(len) words
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t len, uint32_t* ws, hipStream_t s) {
+    return exclusive_scan(ScanArray{in}, out, len, ws, s);
+}
+
+// first[c] = exclusive scan of the cell counts; workspace as above
+int exclusive_scan_cell_counts(const uint32_t* cellStart, const uint32_t* cellEnd, uint32_t* first,
+                               size_t numCells, uint32_t* ws, hipStream_t s) {
+    return exclusive_scan(ScanCellCounts{cellStart, cellEnd}, first, numCells, ws, s);
 }
 
 // ---- radix passes ----------------------------------------------------------
