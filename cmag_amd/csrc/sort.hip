@@ -201,9 +201,11 @@ int exclusive_scan_cell_counts(const uint32_t* cellStart, const uint32_t* cellEn
 }
 
 // ---- radix passes ----------------------------------------------------------
-// per-tile digit counts: each wave counts into its own LDS histogram with
-// plain LDS atomics (lanes that hit one bin are serialised by the hardware),
-// and the block adds the wave histograms
+// per-tile digit counts: each wave counts into its own LDS histogram, and the
+// block adds the wave histograms.  Keys in particle order are spatially
+// coherent, so neighbouring lanes often share a digit: only the first lane of
+// each run of equal digits adds, the run's length (lanes of one bin in
+// different runs are serialised by the LDS atomics)
 template <int ITEMS>
 __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restrict__ keys, uint32_t n,
                                                          int shift, uint32_t* __restrict__ counts,
@@ -221,9 +223,17 @@ __global__ __launch_bounds__(RS_NT) void rs_hist_kernel(const uint32_t* __restri
         const uint32_t idx = base + k * RS_NT + tid;
         d[k] = idx < n ? (keys[idx] >> shift) & 255u : 256u;
     }
+    const int lane = __lane_id();
+    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k)
-        if (d[k] < 256u) atomicAdd(&hist[w][d[k]], 1u);
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t prev = __shfl_up(d[k], 1);
+        const bool head = lane == 0 || prev != d[k];
+        const uint64_t heads = __ballot(head);
+        const uint64_t later = heads & above;
+        const uint32_t next = later ? (uint32_t)(__ffsll((unsigned long long)later) - 1) : 64u;
+        if (head && d[k] < 256u) atomicAdd(&hist[w][d[k]], next - (uint32_t)lane);
+    }
     __syncthreads();
     uint32_t c = 0;
 #pragma unroll
