@@ -98,16 +98,23 @@ struct ScanCellCounts {
     }
 };
 
+// the block total of one tile; each thread sums SCAN_ITEMS consecutive
+// elements, as 16-byte words when vec and they lie inside len
 template <typename Src>
 __global__ __launch_bounds__(SCAN_NT) void scan_reduce_kernel(const Src src, size_t len,
-                                                               uint32_t* __restrict__ partial) {
+                                                               uint32_t* __restrict__ partial, bool vec) {
     __shared__ uint32_t lds[SCAN_NT / 64];
-    size_t base = (size_t)blockIdx.x * SCAN_TILE;
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
     uint32_t sum = 0;
+    if (vec && base + SCAN_ITEMS <= len) {
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; ++k) {
-        size_t i = base + (size_t)k * SCAN_NT + threadIdx.x;
-        sum += (i < len) ? src.at(i) : 0u;
+        for (int q = 0; q < SCAN_ITEMS / 4; ++q) {
+            const uint4 x = src.at4(base + 4 * q);
+            sum += (x.x + x.y) + (x.z + x.w);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) sum += (base + k < len) ? src.at(base + k) : 0u;
     }
     uint32_t tot;
     block_exclusive_scan<SCAN_NT>(sum, lds, &tot);
@@ -180,7 +187,8 @@ static int exclusive_scan(const Src& src, uint32_t* out, size_t len, uint32_t* w
     }
     size_t blocks = (len + SCAN_TILE - 1) / SCAN_TILE;
     uint32_t* partial = ws;
-    hipLaunchKernelGGL(scan_reduce_kernel<Src>, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, src, len, partial);
+    hipLaunchKernelGGL(scan_reduce_kernel<Src>, dim3((uint32_t)blocks), dim3(SCAN_NT), 0, s, src, len, partial,
+                       vec);
     CMAG_TRY(check_launch("scan_reduce_kernel"));
     // scan the partials in place (recursively for very long inputs)
     CMAG_TRY(exclusive_scan(ScanArray{partial}, partial, blocks, ws + blocks, s));
