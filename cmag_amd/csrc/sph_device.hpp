@@ -526,8 +526,8 @@ struct ForceSum {
 // term would run for every candidate (some lane of the wave is almost always
 // in range), i.e. ~475 times per particle instead of ~74.  The list is an
 // NbQueue of CAP entries per lane (NB_CAP unless a caller picks another
-// depth); pair terms take entries two at a time with all six row loads issued
-// before either term.
+// depth); pair terms take entries two at a time with all their row loads
+// (four in the sub-cell walk, six otherwise) issued before either term.
 // nbl: this wave's queues in LDS, [CAP][64]
 // In the sub-cell walk (Cells::SUB, DamBreak) the row copies carry the
 // density pass's results: a fluid row holds -rho in pos.w, sign bit set (a wall
