@@ -210,7 +210,8 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
     return check_launch("subcell_move_kernel");
 }
 
-// Density in sub-cell order.  The result goes to the reference-order row
+// Density in sub-cell order (cell lookups staged in LDS as in force_sub_kernel).
+// The result goes to the reference-order row
 // measures[w2r[i]] and, for a fluid row, into the sub-cell copies that the
 // force pass reads: -rho in posW[i].w and p / rho^2 in velW[i].w.  Other threads read
 // posW rows during this pass, but only their x, y, z; a thread writes the w of
@@ -219,10 +220,15 @@ template <int DIM, int HASH>
 __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, float4* __restrict__ measures,
                                                            const uint32_t* __restrict__ w2r, float4* posW,
                                                            float4* velW, const SubCells cells, uint32_t n) {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    __shared__ LookupPlan lp;
+    __shared__ LookupTables lt;
+    const uint32_t b0 = xcd_block() * blockDim.x;
+    const uint32_t i = b0 + threadIdx.x;
+    const bool staged = stage_lookups<DIM>(P, posW, cells, n, b0, lp, lt);
     if (i >= n) return;
     const float4 pi = posW[i];
-    const float dens = density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells);
+    const float dens = staged ? density_sum_lookup<DIM>(P, pi, lp, lt, posW)
+                              : density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells);
     const float p = eos(P, dens);
     reinterpret_cast<float2*>(measures)[2 * (size_t)w2r[i]] = make_float2(dens, p);
     if (pi.w == CMAG_DB_FLUID) {
@@ -251,6 +257,26 @@ int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2
     return check_launch("density_sub_kernel");
 }
 
+// Force in sub-cell order; a block whose targets share one row of cells finds
+// its runs through the cell lookups staged in LDS (the same runs as the
+// global walk)
+template <int DIM, int HASH>
+__global__ __launch_bounds__(256) void force_sub_kernel(const cmag_params P, float4* __restrict__ acc,
+                                                         const float4* __restrict__ posW,
+                                                         const float4* __restrict__ velW,
+                                                         const uint32_t* __restrict__ indexW, const SubCells cells,
+                                                         uint32_t n) {
+    __shared__ uint32_t nbl[256 / 64][NB_CAP][64];
+    __shared__ LookupPlan lp;
+    __shared__ LookupTables lt;
+    const uint32_t b0 = xcd_block() * blockDim.x;
+    const uint32_t i = b0 + threadIdx.x;
+    const bool staged = stage_lookups<DIM>(P, posW, cells, n, b0, lp, lt);
+    if (i >= n) return;
+    force_body<CMAG_SYS_DAMBREAK, DIM, HASH>(P, i, acc, nullptr, posW, velW, indexW, cells, n, nbl[threadIdx.x >> 6],
+                                             staged ? &lp : nullptr, &lt);
+}
+
 int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const float* velW,
                      const uint32_t* indexW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
                      hipStream_t s) {
@@ -259,9 +285,9 @@ int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const 
     if (P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_INVALID, "the sub-cell walk is DamBreak's");
     const SubCells cells{cellFirst, reinterpret_cast<const uint4*>(pre)};
     auto run = [&](auto dim, auto hash) {
-        hipLaunchKernelGGL((force_kernel<CMAG_SYS_DAMBREAK, decltype(dim)::value, decltype(hash)::value, SubCells>),
+        hipLaunchKernelGGL((force_sub_kernel<decltype(dim)::value, decltype(hash)::value>),
                            dim3(div_up(n, 256)), dim3(256), 0, s, P, reinterpret_cast<float4*>(acc),
-                           nullptr, reinterpret_cast<const float4*>(posW),
+                           reinterpret_cast<const float4*>(posW),
                            reinterpret_cast<const float4*>(velW), indexW, cells, n);
     };
     const bool m = P.hash_order == CMAG_HASH_MORTON;
@@ -271,7 +297,7 @@ int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const 
     using H1 = std::integral_constant<int, 1>;
     if (P.dim == 3) { if (m) run(I3{}, H1{}); else run(I3{}, H0{}); }
     else { if (m) run(I2{}, H1{}); else run(I2{}, H0{}); }
-    return check_launch("force_kernel");
+    return check_launch("force_sub_kernel");
 }
 
 template <bool V, bool PR, typename Cells>

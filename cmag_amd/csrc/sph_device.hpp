@@ -285,9 +285,37 @@ __device__ __forceinline__ void walk_stencil(const cmag_params& P, const float4&
 // (indices clamped to the run's last row, and the second call flagged
 // valid = false when it lies past the run: the candidate must then contribute
 // nothing), which keeps the loads out of divergent branches.
+// the rows [s, e) of a sub-cell copy as one loop over a byte offset, two rows
+// per iteration, each load issued one candidate ahead of its use; it reads up
+// to two rows past the run (the copies carry padding rows)
+template <typename C>
+__device__ __forceinline__ void walk_run_sub(const float4* __restrict__ pos, uint32_t s, uint32_t e, float px,
+                                             float py, float pz, int code, C&& cand) {
+    const char* base = reinterpret_cast<const char*>(pos);
+    auto ld = [&](uint32_t off) { return *reinterpret_cast<const float4*>(base + off); };
+    const uint32_t end = e << 4, end1 = end - 16u;
+    uint32_t off = s << 4;
+    if (off >= end) return;
+    float4 p0 = ld(off);
+    do {
+        const float4 p1 = ld(off + 16u);
+        cand(off >> 4, p0, true, px, py, pz, code);
+        p0 = ld(off + 32u);
+        cand((off >> 4) + 1u, p1, off < end1, px, py, pz, code);
+        off += 32u;
+    } while (off < end);
+}
+
 template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
 __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const float4& pi, const Cell& gp,
                                                 const Cells& cells, const float4* __restrict__ pos, C&& cand) {
+    if constexpr (Cells::SUB) {
+        walk_rows<DIM, HASH, PERIODIC>(P, pi, gp, cells, [&](const Run& a, const Run& b, float py, float pz) {
+            walk_run_sub(pos, a.s, a.s + a.len, a.px, py, pz, a.code, cand);
+            walk_run_sub(pos, b.s, b.s + b.len, b.px, py, pz, b.code, cand);
+        });
+        return;
+    }
     walk_rows<DIM, HASH, PERIODIC>(P, pi, gp, cells, [&](const Run& a, const Run& b, float py, float pz) {
         const uint32_t total = a.len + b.len;
         if (total == 0) return;
@@ -310,6 +338,87 @@ __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const floa
             visit(t1, j1, p1, t + 1u < total);
         }
     });
+}
+
+// ---- cell lookups staged in LDS (sub-cell walk) -----------------------------
+// A block whose targets all lie in one row of cells along x, with the
+// stencil's x-range [cx0 - 1, cx1 + 1] inside the grid, on consecutive keys
+// and at most LK_W cells wide, copies the compact offsets and quarter counts
+// of those cells in its 9 (2-D: 3) stencil rows into LDS once; its targets
+// then find their runs there instead of with five loads per stencil row.
+// The runs and their order are the global walk's, so the sums are the same
+// bit for bit.
+constexpr int LK_W = 32;
+
+struct LookupPlan {
+    uint32_t key[9];  // key of cell cx0 - 1 in each stencil row
+    uint32_t width;   // cells per stencil row segment
+    int cx0;
+    int ok;
+};
+
+struct LookupTables {
+    uint32_t first[9][LK_W + 1];
+    uint4 pre[9][LK_W];
+};
+
+// all threads of the block call this; returns whether the block's lookups are staged
+template <int DIM>
+__device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4* __restrict__ pos,
+                                              const SubCells& cells, uint32_t n, uint32_t b0, LookupPlan& lp,
+                                              LookupTables& lt) {
+    constexpr int R = DIM == 3 ? 9 : 3;
+    if (threadIdx.x == 0) {
+        int ok = 0;
+        if (b0 < n && P.cellcount == 1) {
+            const uint32_t b1 = min(b0 + blockDim.x, n) - 1u;
+            const float4 pa = pos[b0], pb = pos[b1];
+            const Cell a = grid_pos(P, pa.x, pa.y, pa.z), b = grid_pos(P, pb.x, pb.y, pb.z);
+            const int gx = (int)P.grid_size[0], gy = (int)P.grid_size[1], gz = (int)P.grid_size[2];
+            const int ax = a.x & (gx - 1), ay = a.y & (gy - 1), az = a.z & (gz - 1);
+            const int bx = b.x & (gx - 1), by = b.y & (gy - 1), bz = b.z & (gz - 1);
+            if (ay == by && az == bz && ax >= 1 && bx >= ax && bx + 1 < gx && bx - ax + 3 <= LK_W) {
+                const uint32_t width = (uint32_t)(bx - ax) + 3u;
+                if (grid_hash(P, bx + 1, ay, az) - grid_hash(P, ax - 1, ay, az) == width - 1u) {
+                    int r = 0;
+                    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+                        for (int dy = -1; dy <= 1; ++dy, ++r) lp.key[r] = grid_hash(P, ax - 1, ay + dy, az + dz);
+                    lp.width = width;
+                    lp.cx0 = ax;
+                    ok = 1;
+                }
+            }
+        }
+        lp.ok = ok;
+    }
+    __syncthreads();
+    if (!lp.ok) return false;
+    const uint32_t width = lp.width;
+    for (uint32_t t = threadIdx.x; t < (uint32_t)R * (LK_W + 1); t += blockDim.x) {
+        const uint32_t r = t / (LK_W + 1), c = t % (LK_W + 1);
+        if (c <= width) lt.first[r][c] = cells.first[lp.key[r] + c];
+        if (c < width) lt.pre[r][c] = cells.pre[lp.key[r] + c];
+    }
+    __syncthreads();
+    return true;
+}
+
+// the candidates of target pi (cell x wx, wrapped) through the staged
+// lookups, as walk_stencil_pf hands them
+template <int DIM, typename C>
+__device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, int wx, const LookupPlan& lp,
+                                            const LookupTables& lt, const float4* __restrict__ pos, C&& cand) {
+    constexpr int R = DIM == 3 ? 9 : 3;
+    const int q = x_quarter(P, pi.x);
+    const uint32_t dx = (uint32_t)(wx - lp.cx0);
+    for (int r = 0; r < R; ++r) {
+        uint32_t s = lt.first[r][dx];
+        if (q > 0) s += SubCells::through(lt.pre[r][dx], q - 1);
+        const uint32_t f1 = lt.first[r][dx + 2];
+        const uint4 c1 = lt.pre[r][dx + 2];
+        walk_run_sub(pos, s, f1 + SubCells::through(c1, q), pi.x, pi.y, pi.z, 13, cand);
+        walk_run_sub(pos, f1 + c1.w, lt.first[r][dx + 3], pi.x, pi.y, pi.z, 13, cand);
+    }
 }
 
 constexpr int NB_CAP = 16;
@@ -378,6 +487,22 @@ __device__ __forceinline__ float density_sum(const cmag_params& P, const float4&
         // t = 1 - q/2 with q = r/h, clamped at 0 (nothing beyond 2h, no
         // branch); 1 - q/2 <= 1, so the clamp to [0, 1] is max(0, .) and folds
         // into the fma.  (1 - q/2)^4 (2q + 1) = t^4 (5 - 4t)
+        const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
+        const float t2 = t * t;
+        sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
+    });
+    return sum * P.kernel_coeff * P.particle_mass;
+}
+
+template <int DIM>
+__device__ __forceinline__ float density_sum_lookup(const cmag_params& P, const float4& pi, const LookupPlan& lp,
+                                                    const LookupTables& lt, const float4* __restrict__ pos) {
+    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
+    const float half_inv_h = -0.5f / P.smoothing_radius;
+    float sum = 0.0f;
+    walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), lp, lt, pos,
+                     [&](uint32_t, const float4& pj, bool valid, float px, float py, float pz, int) {
+        const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
         const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
         const float t2 = t * t;
         sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
@@ -539,7 +664,8 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                                            const float4* __restrict__ sortedPos,
                                            const float4* __restrict__ sortedVel,
                                            const uint32_t* __restrict__ index, const Cells& cells, uint32_t n,
-                                           uint32_t (*nbl)[64]) {
+                                           uint32_t (*nbl)[64], const LookupPlan* lp = nullptr,
+                                           const LookupTables* lt = nullptr) {
     const int lane = threadIdx.x & 63;
     constexpr bool PACKED = SYS == CMAG_SYS_DAMBREAK && Cells::SUB;
     auto is_wall = [](float w) { return PACKED ? __float_as_int(w) >= 0 : w != CMAG_DB_FLUID; };
@@ -581,9 +707,7 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         }
     };
 
-    walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos,
-                                                         [&](uint32_t j, const float4& pj, bool valid, float px,
-                                                             float py, float pz, int code) {
+    auto cand = [&](uint32_t j, const float4& pj, bool valid, float px, float py, float pz, int code) {
         const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
         const float r2 = norm2(rx, ry, rz);
         if (valid && j != i) {
@@ -595,7 +719,13 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
             }
         }
         if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
-    });
+    };
+    if constexpr (Cells::SUB) {
+        if (lp) walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), *lp, *lt, sortedPos, cand);
+        else walk_stencil_pf<DIM, HASH, false>(P, pi, gp, cells, sortedPos, cand);
+    } else {
+        walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos, cand);
+    }
     nq.drain(CAP, pairs);
     const uint32_t k = index[i];
     if (k < n) acc[k] = make_float4(F.fx + F.wx, F.fy + F.wy, F.fz + F.wz, 0.0f);

@@ -144,7 +144,8 @@ struct cmag_system {
         if (on && !posW) {
             const size_t f4 = sizeof(float) * 4 * (size_t)(n ? n : 1);
             const size_t u = sizeof(uint32_t) * (size_t)(n ? n : 1);
-            for (float** b : {&posW, &velW}) CMAG_HIP_TRY(hipMalloc((void**)b, f4));
+            // padding rows: the walk reads up to two rows past a run
+            for (float** b : {&posW, &velW}) CMAG_HIP_TRY(hipMalloc((void**)b, f4 + 64));
             for (uint32_t** b : {&indexW, &w2r}) CMAG_HIP_TRY(hipMalloc((void**)b, u));
             // padded to whole 16-byte words: the move kernel reads buckets 16 at a time
             CMAG_HIP_TRY(hipMalloc((void**)&bucketR, ((size_t)n + 31) & ~(size_t)15));
