@@ -403,21 +403,73 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
     return true;
 }
 
+// The density pass also stages candidate rows: after the lookups, the rows of
+// the block's three middle stencil rows of cells (dz = 0 in 3-D), up to
+// ROW_STAGE_CAP rows, are copied into LDS and read from there.  16 KB keeps
+// seven blocks per CU; more rows cost more in occupancy than they save in
+// loads, and the force pass has no LDS to spare (its queues take 16 KB).
+constexpr int ROW_STAGE_FIRST = 3, ROW_STAGE_ROWS = 3, ROW_STAGE_CAP = 1024;
+
+struct RowStage {
+    int32_t off[ROW_STAGE_ROWS];  // LDS row of sorted row j of staged stencil row r: j + off
+    int ok;
+};
+
+// all threads of a staged block call this after stage_lookups; lds holds
+// ROW_STAGE_CAP + 1 rows (the walk reads one row past a run)
+template <int DIM>
+__device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTables& lt,
+                                           const float4* __restrict__ pos, float4* lds, RowStage& rs) {
+    if (threadIdx.x == 0) {
+        int32_t tot = 0;
+        for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
+            const uint32_t s = lt.first[ROW_STAGE_FIRST + k][0], e = lt.first[ROW_STAGE_FIRST + k][lp.width];
+            rs.off[k] = tot - (int32_t)s;
+            tot += (int32_t)(e - s);
+        }
+        rs.ok = DIM == 3 && tot <= ROW_STAGE_CAP;
+    }
+    __syncthreads();
+    if (!rs.ok) return;
+    for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
+        const uint32_t s = lt.first[ROW_STAGE_FIRST + k][0], e = lt.first[ROW_STAGE_FIRST + k][lp.width];
+        float4* dst = lds + ((int32_t)s + rs.off[k]);
+        for (uint32_t t = threadIdx.x; t < e - s; t += blockDim.x) dst[t] = pos[s + t];
+    }
+    __syncthreads();
+}
+
 // the candidates of target pi (cell x wx, wrapped) through the staged
-// lookups, as walk_stencil_pf hands them
-template <int DIM, typename C>
+// lookups, as walk_stencil_pf hands them; with ROWS, the stencil rows that
+// stage_rows copied are read from LDS
+template <int DIM, bool ROWS = false, typename C>
 __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, int wx, const LookupPlan& lp,
-                                            const LookupTables& lt, const float4* __restrict__ pos, C&& cand) {
+                                            const LookupTables& lt, const float4* __restrict__ pos, C&& cand,
+                                            const RowStage* rs = nullptr, const float4* lds = nullptr) {
     constexpr int R = DIM == 3 ? 9 : 3;
     const int q = x_quarter(P, pi.x);
     const uint32_t dx = (uint32_t)(wx - lp.cx0);
+    auto run_lds = [&](uint32_t s, uint32_t e, int32_t off) {
+        for (uint32_t j = s; j < e; j += 2u) {
+            const int32_t l = (int32_t)j + off;  // in [0, staged rows]
+            const float4 p0 = lds[l], p1 = lds[l + 1];
+            cand(j, p0, true, pi.x, pi.y, pi.z, 13);
+            cand(j + 1u, p1, j + 1u < e, pi.x, pi.y, pi.z, 13);
+        }
+    };
     for (int r = 0; r < R; ++r) {
         uint32_t s = lt.first[r][dx];
         if (q > 0) s += SubCells::through(lt.pre[r][dx], q - 1);
         const uint32_t f1 = lt.first[r][dx + 2];
         const uint4 c1 = lt.pre[r][dx + 2];
-        walk_run_sub(pos, s, f1 + SubCells::through(c1, q), pi.x, pi.y, pi.z, 13, cand);
-        walk_run_sub(pos, f1 + c1.w, lt.first[r][dx + 3], pi.x, pi.y, pi.z, 13, cand);
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt.first[r][dx + 3];
+        if (ROWS && rs->ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS) {
+            run_lds(s, e, rs->off[r - ROW_STAGE_FIRST]);
+            run_lds(ws, we, rs->off[r - ROW_STAGE_FIRST]);
+        } else {
+            walk_run_sub(pos, s, e, pi.x, pi.y, pi.z, 13, cand);
+            walk_run_sub(pos, ws, we, pi.x, pi.y, pi.z, 13, cand);
+        }
     }
 }
 
@@ -496,17 +548,18 @@ __device__ __forceinline__ float density_sum(const cmag_params& P, const float4&
 
 template <int DIM>
 __device__ __forceinline__ float density_sum_lookup(const cmag_params& P, const float4& pi, const LookupPlan& lp,
-                                                    const LookupTables& lt, const float4* __restrict__ pos) {
+                                                    const LookupTables& lt, const RowStage& rs, const float4* lds,
+                                                    const float4* __restrict__ pos) {
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     float sum = 0.0f;
-    walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), lp, lt, pos,
-                     [&](uint32_t, const float4& pj, bool valid, float px, float py, float pz, int) {
+    walk_lookup<DIM, true>(P, pi, gp.x & ((int)P.grid_size[0] - 1), lp, lt, pos,
+                           [&](uint32_t, const float4& pj, bool valid, float px, float py, float pz, int) {
         const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
         const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
         const float t2 = t * t;
         sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
-    });
+    }, &rs, lds);
     return sum * P.kernel_coeff * P.particle_mass;
 }
 
