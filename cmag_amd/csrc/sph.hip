@@ -211,7 +211,7 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
 }
 
 // Density in sub-cell order (cell lookups staged in LDS as in force_sub_kernel,
-// and the candidate rows of the middle stencil rows as well).
+// and the candidate positions of four stencil rows as well).
 // The result goes to the reference-order row
 // measures[w2r[i]] and, for a fluid row, into the sub-cell copies that the
 // force pass reads: -rho in posW[i].w and p / rho^2 in velW[i].w.  Other threads read
@@ -226,12 +226,12 @@ __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, f
     const uint32_t b0 = xcd_block() * blockDim.x;
     const uint32_t i = b0 + threadIdx.x;
     __shared__ RowStage rs;
-    __shared__ float4 rows[ROW_STAGE_CAP + 1];
+    __shared__ RowLds rows;
     const bool staged = stage_lookups<DIM>(P, posW, cells, n, b0, lp, lt);
     if (staged) stage_rows<DIM>(lp, lt, posW, rows, rs);
     if (i >= n) return;
     const float4 pi = posW[i];
-    const float dens = staged ? density_sum_lookup<DIM>(P, pi, lp, lt, rs, rows, posW)
+    const float dens = staged ? density_sum_lookup<DIM>(P, pi, lp, lt, rs, &rows, posW)
                               : density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells);
     const float p = eos(P, dens);
     reinterpret_cast<float2*>(measures)[2 * (size_t)w2r[i]] = make_float2(dens, p);

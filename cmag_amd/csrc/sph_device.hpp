@@ -403,23 +403,29 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
     return true;
 }
 
-// The density pass also stages candidate rows: after the lookups, the rows of
-// the block's three middle stencil rows of cells (dz = 0 in 3-D), up to
-// ROW_STAGE_CAP rows, are copied into LDS and read from there.  16 KB keeps
-// seven blocks per CU; more rows cost more in occupancy than they save in
-// loads, and the force pass has no LDS to spare (its queues take 16 KB).
-constexpr int ROW_STAGE_FIRST = 3, ROW_STAGE_ROWS = 3, ROW_STAGE_CAP = 1024;
+// The density pass also stages candidate rows: after the lookups, the
+// positions of the rows of four stencil rows of cells (r = 3..6: the three of
+// dz = 0 and the first of dz = +1 in 3-D), up to ROW_STAGE_CAP rows, are copied
+// into LDS as (x, y) pairs and z (12 B a row, 16 KB) and read from there.
+// 16 KB keeps seven blocks per CU; more rows cost more in occupancy than they
+// save in loads, and the force pass has no LDS to spare (its queues take 16 KB).
+constexpr int ROW_STAGE_FIRST = 3, ROW_STAGE_ROWS = 4, ROW_STAGE_CAP = 1365;
 
 struct RowStage {
     int32_t off[ROW_STAGE_ROWS];  // LDS row of sorted row j of staged stencil row r: j + off
     int ok;
 };
 
-// all threads of a staged block call this after stage_lookups; lds holds
-// ROW_STAGE_CAP + 1 rows (the walk reads one row past a run)
+// ROW_STAGE_CAP + 1 rows: the walk reads one row past a run
+struct RowLds {
+    float2 xy[ROW_STAGE_CAP + 1];
+    float z[ROW_STAGE_CAP + 1];
+};
+
+// all threads of a staged block call this after stage_lookups
 template <int DIM>
 __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTables& lt,
-                                           const float4* __restrict__ pos, float4* lds, RowStage& rs) {
+                                           const float4* __restrict__ pos, RowLds& lds, RowStage& rs) {
     if (threadIdx.x == 0) {
         int32_t tot = 0;
         for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
@@ -433,8 +439,12 @@ __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTab
     if (!rs.ok) return;
     for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
         const uint32_t s = lt.first[ROW_STAGE_FIRST + k][0], e = lt.first[ROW_STAGE_FIRST + k][lp.width];
-        float4* dst = lds + ((int32_t)s + rs.off[k]);
-        for (uint32_t t = threadIdx.x; t < e - s; t += blockDim.x) dst[t] = pos[s + t];
+        const int32_t d0 = (int32_t)s + rs.off[k];
+        for (uint32_t t = threadIdx.x; t < e - s; t += blockDim.x) {
+            const float4 v = pos[s + t];
+            lds.xy[d0 + t] = make_float2(v.x, v.y);
+            lds.z[d0 + t] = v.z;
+        }
     }
     __syncthreads();
 }
@@ -445,14 +455,16 @@ __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTab
 template <int DIM, bool ROWS = false, typename C>
 __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, int wx, const LookupPlan& lp,
                                             const LookupTables& lt, const float4* __restrict__ pos, C&& cand,
-                                            const RowStage* rs = nullptr, const float4* lds = nullptr) {
+                                            const RowStage* rs = nullptr, const RowLds* lds = nullptr) {
     constexpr int R = DIM == 3 ? 9 : 3;
     const int q = x_quarter(P, pi.x);
     const uint32_t dx = (uint32_t)(wx - lp.cx0);
     auto run_lds = [&](uint32_t s, uint32_t e, int32_t off) {
         for (uint32_t j = s; j < e; j += 2u) {
             const int32_t l = (int32_t)j + off;  // in [0, staged rows]
-            const float4 p0 = lds[l], p1 = lds[l + 1];
+            const float2 a0 = lds->xy[l], a1 = lds->xy[l + 1];
+            const float4 p0 = make_float4(a0.x, a0.y, lds->z[l], 0.0f);
+            const float4 p1 = make_float4(a1.x, a1.y, lds->z[l + 1], 0.0f);
             cand(j, p0, true, pi.x, pi.y, pi.z, 13);
             cand(j + 1u, p1, j + 1u < e, pi.x, pi.y, pi.z, 13);
         }
@@ -548,7 +560,7 @@ __device__ __forceinline__ float density_sum(const cmag_params& P, const float4&
 
 template <int DIM>
 __device__ __forceinline__ float density_sum_lookup(const cmag_params& P, const float4& pi, const LookupPlan& lp,
-                                                    const LookupTables& lt, const RowStage& rs, const float4* lds,
+                                                    const LookupTables& lt, const RowStage& rs, const RowLds* lds,
                                                     const float4* __restrict__ pos) {
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float half_inv_h = -0.5f / P.smoothing_radius;
