@@ -30,6 +30,17 @@
  *       field gives a = -grad(p)/rho and a quadratic shear flow
  *       u = y^2 gives the Morris viscous acceleration 2*nu; Lennard-Jones term
  *       against a single wall particle in closed form; O(N^2) brute force.
+ *       DamBreak (tests/test_oracle_pins.py): the linear-pressure consistency
+ *       check on 3-D and 2-D lattices (h = 2dx and 1.3dx) fixes the gradient
+ *       normalisations 21/(8 pi h^4) and 7/(2 pi h^3); two-particle closed
+ *       forms with exact inputs fix the Monaghan term (nu = 2 alpha h c /
+ *       (rho_i + rho_j), the 0.001 h^2 regulariser, zero for separating
+ *       pairs); the viscosity conserves momentum and dissipates energy.
+ *       Every such pin rejects a planted mistake (ORACLE_MUTATE builds).
+ *   Tait equation of state
+ *       closed forms p(rho0) = 0, p(2^(1/7) rho0) = B, p(2 rho0) = 127 B and
+ *       dp/drho(rho0) = B gamma / rho0, densities set through an isolated
+ *       particle's own kernel value; planted exponent and B mistakes fail.
  *   oracle_integrate_* / configure_boundary / shift / remove
  *       closed-form single-particle updates with dyadic numbers (exact in
  *       binary), wrap and clamp cases, the analytic wall sinusoid.

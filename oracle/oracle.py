@@ -6,6 +6,7 @@ what each restated function follows in the reference and how it is pinned.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import subprocess
 from pathlib import Path
@@ -25,17 +26,45 @@ def build() -> Path:
     return LIB_PATH
 
 
+def _load(path: Path) -> C.CDLL:
+    lb = C.CDLL(str(path))
+    lb.oracle_grid_hash.restype = C.c_uint32
+    lb.oracle_dambreak_count.restype = C.c_uint32
+    lb.oracle_peristalsis_mass_sum.restype = C.c_float
+    lb.oracle_get_threads.restype = C.c_int
+    return lb
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        _lib = C.CDLL(str(LIB_PATH))
-        _lib.oracle_grid_hash.restype = C.c_uint32
-        _lib.oracle_dambreak_count.restype = C.c_uint32
-        _lib.oracle_peristalsis_mass_sum.restype = C.c_float
-        _lib.oracle_get_threads.restype = C.c_int
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def build_variant(mutation: int, outdir) -> Path:
+    """The oracle built with the planted mistake ORACLE_MUTATE=mutation into
+    outdir (a scratch directory): the pin tests check that they reject it."""
+    out = Path(outdir).resolve() / f"liboracle_mut{int(mutation)}.so"
+    proc = subprocess.run(["make", "-s", "-C", str(HERE), f"OUT={out}", f"DEFS=-DORACLE_MUTATE={int(mutation)}"],
+                          capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"oracle variant build failed:\n{proc.stdout}\n{proc.stderr}")
+    return out
+
+
+@contextlib.contextmanager
+def using(path):
+    """Every function of this module calls the library at path inside the block."""
+    global _lib
+    saved = lib()
+    _lib = _load(Path(path))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _ptr(a: np.ndarray | None):
