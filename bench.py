@@ -254,14 +254,25 @@ def bench_adjoint(reps: int = 3) -> dict:
 
 
 def cpu_baseline(target_seconds: float = 12.0) -> dict:
+    """The oracle's FP32 update() (the reference's loop bodies restated in C,
+    OpenMP over particles) on the 1M-fluid dam break, timed on this host.
+    Threads: the cores this process may run on, at most 16 -- the host-core
+    share one GPU's job gets on the GPU hosts (OMP_NUM_THREADS is 16 there);
+    nproc and the affinity count are reported beside it.  The library is the
+    AVX-512 build (x86-64-v4) when the host has AVX-512, else the AVX2 one."""
     from cmag_amd import scenarios as S
     from oracle import oracle as O
 
     try:
-        cores = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        affinity = os.cpu_count() or 1
+    cores = max(1, min(affinity, 16))
+    isa = "x86-64-v3"
+    v4 = O.HERE / "liboracle_v4.so"
+    if v4.exists() and O.host_has_avx512():
+        O.use_default(v4)
+        isa = "x86-64-v4"
     O.set_threads(cores)
     sc = S.dambreak3d_particles(1_024_000, hash_order=1)
     sim = O.Simulation(sc.params, sc.pos, sc.vel)
@@ -277,10 +288,13 @@ def cpu_baseline(target_seconds: float = 12.0) -> dict:
         "value": sc.num_fluid * steps / el,
         "unit": "particle-steps/s",
         "cores": cores,
+        "nproc": os.cpu_count(),
+        "affinity_cores": affinity,
         "kind": "port",
+        "isa": isa,
         "sample": (f"oracle FP32 update() of the 3-D dam break with {sc.num_fluid:,} fluid + "
                    f"{sc.meta['num_wall']:,} wall particles, {steps} steps after 1 warm-up step, "
-                   f"{cores} OpenMP threads"),
+                   f"{cores} OpenMP threads, {isa} build"),
     }
 
 
@@ -353,6 +367,8 @@ def main() -> int:
         copy_probe()
     wall_s = t1 - t0
     ms_per_step = 1000.0 * wall_s / args.steps
+    vmax_after = round(float(np.linalg.norm(sysobj.getArray(L.ARRAY_VELOCITY, 0, sc.num_fluid)[:, :3],
+                                            axis=1).max()), 3)
 
     per_stage = {}
     for name, (ms, cnt) in stages.items():
@@ -384,7 +400,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"3-D dam break, {sc.num_fluid:,} fluid + {sc.meta['num_wall']:,} wall particles, "
-                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {hash_name} cell hash, Tait, alpha=0.01"),
+                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {hash_name} cell hash, Tait, alpha=0.01; "
+                         f"the column from rest behind its gate (first {args.warmup + args.steps} steps)"),
+            "fluid_vmax_after": vmax_after,
             "num_fluid": sc.num_fluid,
             "num_total": n_total,
             "grid": list(sc.params.grid_size),

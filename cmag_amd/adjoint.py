@@ -32,7 +32,8 @@ def _f4(a, device) -> torch.Tensor:
 class ChannelRollout:
     """A Poiseuille system with a fixed initial state and step count; calling
     it with (mu, gravity) tensors returns the final (pos, vel) as a
-    differentiable function of them."""
+    differentiable function of them (losses may depend on both; the type flag
+    in pos[:, 3] carries no gradient)."""
 
     def __init__(self, params, pos, vel=None, vel_lf=None, steps: int = 200, checkpoint_every: int = 20,
                  use_graph: bool = True):
@@ -53,10 +54,13 @@ class ChannelRollout:
         vel_lf = vel if vel_lf is None else vel_lf
         self.state0 = (_f4(pos, dev), _f4(vel, dev), _f4(vel_lf, dev))
         self.system = ParamSystem(self.params, self.n)
-        ncp = (self.steps + self.checkpoint_every - 1) // self.checkpoint_every
-        mk = lambda: tuple(torch.empty((self.n, 4), dtype=torch.float32, device=dev) for _ in range(3))  # noqa: E731
-        self._cps = [mk() for _ in range(ncp)]
-        self._seg = [mk() for _ in range(self.checkpoint_every)]
+        self._ncp = (self.steps + self.checkpoint_every - 1) // self.checkpoint_every
+        # the recomputed states of one segment: scratch of a single backward
+        # pass (autograd runs backward passes one after another)
+        self._seg = [self._state() for _ in range(self.checkpoint_every)]
+
+    def _state(self):
+        return tuple(torch.empty((self.n, 4), dtype=torch.float32, device=self.device) for _ in range(3))
 
     def __call__(self, mu: torch.Tensor, gravity: torch.Tensor):
         return _Rollout.apply(mu, gravity, self)
@@ -81,16 +85,21 @@ class ChannelRollout:
 
 
 class _Rollout(torch.autograd.Function):
+    """Each forward call keeps its own checkpoints on ctx, so several calls of
+    one rollout (two parameter points in one loss) backpropagate through their
+    own trajectories."""
+
     @staticmethod
     def forward(ctx, mu, gravity, run: ChannelRollout):
         g = gravity.detach().reshape(3).cpu().tolist()
         run._params(float(mu.detach().cpu()), g)
         s = run.system
+        cps = [run._state() for _ in range(run._ncp)]
         torch.cuda.synchronize()
         run._put(run.state0)
         t, c = 0, 0
         while t < run.steps:
-            run._take(run._cps[c])
+            run._take(cps[c])
             k = min(run.checkpoint_every, run.steps - t)
             s.run(k, use_graph=run.use_graph)
             t += k
@@ -101,10 +110,10 @@ class _Rollout(torch.autograd.Function):
         s.copy_to_device(L.ARRAY_VELOCITY, vel.data_ptr())
         s.synchronize()
         ctx.run = run
+        ctx.cps = cps
         ctx.theta = (float(mu.detach().cpu()), g)
         ctx.mu_like = (mu.dtype, mu.device, mu.shape)
         ctx.g_like = (gravity.dtype, gravity.device, gravity.shape)
-        ctx.mark_non_differentiable(pos)
         return pos, vel
 
     @staticmethod
@@ -113,18 +122,25 @@ class _Rollout(torch.autograd.Function):
         run._params(*ctx.theta)
         s = run.system
         n, dev = run.n, run.device
-        gpos = torch.zeros((n, 4), dtype=torch.float32, device=dev)
-        gvel = (torch.zeros_like(gpos) if gvel_out is None
-                else gvel_out.detach().to(torch.float32).contiguous().clone())
+
+        def seed(g):
+            # the adjoint of the final state; w (the type flag) is not a variable
+            out = torch.zeros((n, 4), dtype=torch.float32, device=dev)
+            if g is not None:
+                out[:, :3] = g.detach()[:, :3].to(torch.float32)
+            return out
+
+        gpos = seed(gpos_out)
+        gvel = seed(gvel_out)
         glf = torch.zeros_like(gpos)
         gmu = torch.zeros(n, dtype=torch.float64, device=dev)
         ggrav = torch.zeros((n, 4), dtype=torch.float64, device=dev)
         torch.cuda.synchronize()
         ce = run.checkpoint_every
-        for c in reversed(range(len(run._cps))):
+        for c in reversed(range(len(ctx.cps))):
             t0 = c * ce
             t1 = min(t0 + ce, run.steps)
-            run._put(run._cps[c])
+            run._put(ctx.cps[c])
             for t in range(t0, t1):
                 run._take(run._seg[t - t0])
                 if t < t1 - 1:

@@ -105,9 +105,12 @@ def build_examples() -> list[Path]:
 def build_oracle() -> Path:
     """The oracle is test infrastructure; built beside the library for the tests."""
     odir = ROOT / "oracle"
-    proc = subprocess.run(["make", "-s", "-C", str(odir)], capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
+    # the AVX2 library the tests load, and the AVX-512 one the CPU baseline
+    # uses on hosts that have it
+    for extra in ([], ["OUT=liboracle_v4.so", "MARCH=x86-64-v4"]):
+        proc = subprocess.run(["make", "-s", "-C", str(odir)] + extra, capture_output=True, text=True)
+        if proc.returncode != 0:
+            raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
     return odir / "liboracle.so"
 
 

@@ -189,7 +189,12 @@ __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __res
     }
     const uint32_t w = a + below;
     if (w >= n) return;  // inconsistent cell arrays only
-    posW[w] = sortedPos[i];
+    // the walk tells walls from fluid by the sign bit of w (the density pass
+    // stores -rho in fluid rows): every non-fluid row, whatever its type code
+    // (-0.0 and negative codes included), gets the non-negative tag +0
+    float4 p = sortedPos[i];
+    if (p.w != CMAG_DB_FLUID) p.w = 0.0f;
+    posW[w] = p;
     velW[w] = sortedVel[i];
     indexW[w] = index[i];
     w2r[w] = i;

@@ -260,7 +260,7 @@ def poiseuille_series(y: np.ndarray, t: float, F: float, l: float, nu: float,
 def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), layers: int = 3,
                h_factor: float = 1.3, alpha: float = 0.01, jitter: float = 0.01, seed: int = 1973,
                grid=None, hash_order: int = L.HASH_LINEAR, cell_factor: float = 1.0,
-               morton_xbits: int = 0) -> Scenario:
+               morton_xbits: int = 0, gate: bool = True) -> Scenario:
     """3-D dam break in the DamBreak system's loop bodies (Tait EOS, Monaghan
     artificial viscosity, Lennard-Jones wall particles, clamp walls).
 
@@ -268,7 +268,15 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
     tank tank[0] x tank[1] x tank[2]; `layers` wall layers on the floor and the
     four sides (open top); lattice spacing dx with seeded jitter +-jitter*dx.
     Cells are cell_factor * 2h wide (cellcount = ceil(2h / cell)); with the
-    Morton hash, morton_xbits > 0 orders bricks of 2^xbits cells along x."""
+    Morton hash, morton_xbits > 0 orders bricks of 2^xbits cells along x.
+
+    gate: the tank's right wall starts at the column's face as a gate of
+    movable wall particles (type RightFirst, as DamBreak.Core/fluidSystem.cpp's
+    right boundary) with the clamp right_boundary there too, so the column can
+    relax before it is released; changeRightBoundary moves gate and clamp by
+    the column width (fluid_size.x * 2r) per call, and
+    meta["release_shifts"] calls put both at the tank's far wall (release).
+    Without the gate the column starts released."""
     h = h_factor * dx
     cs = cell_factor * 2.0 * h
     cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
@@ -331,21 +339,86 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
         y = -(k + 0.5) * dx
         X, Z = np.meshgrid(xs, zs, indexing="ij")
         walls.append(np.stack([X.ravel(), np.full(X.size, y), Z.ravel()], 1))
-    side_x = np.concatenate([(-np.arange(1, layers + 1) + 0.5) * dx, (nx_t + np.arange(layers) + 0.5) * dx])
-    X, Y, Z = np.meshgrid(side_x, ys, zs, indexing="ij")
-    walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
-    inner_x = (np.arange(0, nx_t) + 0.5) * dx
+    left_x = (-np.arange(1, layers + 1) + 0.5) * dx
+    right_x = (nx_t + np.arange(layers) + 0.5) * dx
     side_z = np.concatenate([(-np.arange(1, layers + 1) + 0.5) * dx, (nz_t + np.arange(layers) + 0.5) * dx])
+    X, Y, Z = np.meshgrid(left_x, ys, zs, indexing="ij")
+    walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
+    inner_z = (np.arange(0, nz_t) + 0.5) * dx
+    if gate:
+        # the right wall's columns beside the tank's z walls stay put; its
+        # inner part is the gate, at the column's face until released
+        X, Y, Z = np.meshgrid(right_x, ys, side_z, indexing="ij")
+        walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
+    else:
+        X, Y, Z = np.meshgrid(right_x, ys, zs, indexing="ij")
+        walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
+    inner_x = (np.arange(0, nx_t) + 0.5) * dx
     X, Y, Z = np.meshgrid(inner_x, ys, side_z, indexing="ij")
     walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
     w = np.concatenate(walls)
     wall = np.empty((w.shape[0], 4), np.float64)
     wall[:, :3] = w
     wall[:, 3] = 1.0  # DamBreak FirstType wall
-    pos = np.concatenate([fluid, wall]).astype(np.float32)
+    parts = [fluid, wall]
+    shifts = 0
+    if gate:
+        shifts = int(round((tank[0] - column[0]) / (nfx * dx)))
+        if abs(shifts * nfx * dx - (tank[0] - column[0])) > 1e-6 * tank[0]:
+            raise ValueError("the gate reaches the tank's far wall only if tank - column is a multiple of column")
+        gx = right_x - shifts * nfx * dx
+        X, Y, Z = np.meshgrid(gx, ys, inner_z, indexing="ij")
+        g4 = np.empty((X.size, 4), np.float64)
+        g4[:, 0], g4[:, 1], g4[:, 2] = X.ravel(), Y.ravel(), Z.ravel()
+        g4[:, 3] = 0.0  # DamBreak RightFirstType: moved by changeRightBoundary
+        parts.append(g4)
+        p.right_boundary = np.float32(nfx * dx)
+    pos = np.concatenate(parts).astype(np.float32)
     vel = np.zeros_like(pos)
-    meta = dict(dx=dx, h=h, c0=c0, tank=tank, column=column, num_wall=int(wall.shape[0]))
+    meta = dict(dx=dx, h=h, c0=c0, tank=tank, column=column, num_wall=int(pos.shape[0] - nf),
+                release_shifts=shifts)
     return Scenario(p, pos, vel, nf, meta)
+
+
+# Martin & Moyce (1952), collapse of a liquid column with n^2 = 2 (height
+# twice the width a): surge front Z = x_front / a against T = t sqrt(2g/a)
+# (their Table 2, as the reference's DamBreak.Report/XFrontTest.h uses it)
+MARTIN_MOYCE_SURGE = ((0.00, 1.00), (0.41, 1.11), (0.84, 1.22), (1.19, 1.44), (1.43, 1.67), (1.63, 1.89),
+                      (1.83, 2.11), (1.98, 2.33), (2.20, 2.56), (2.32, 2.78), (2.51, 3.00), (2.65, 3.22),
+                      (2.83, 3.44), (2.97, 3.67), (3.11, 3.89))
+
+
+def relax_gated(system, sc: Scenario, seconds: float = 0.3, alpha: float = 2.0, zero_every: float = 0.02,
+                free_last: float = 0.02) -> None:
+    """Let a gated dam break settle behind its gate: stepping with a strong
+    artificial viscosity and the velocities set to zero every zero_every
+    seconds (kinetic damping), the last free_last seconds undamped, so that
+    what is left is the column's own residual motion.  The parameter block's
+    alpha is restored afterwards; the clock keeps running."""
+    P0 = system.params
+    P = P0.copy()
+    P.alpha = alpha
+    system.set_params(P)
+    dt = float(P.delta_time)
+    total = int(round(seconds / dt))
+    every = max(1, int(round(zero_every / dt)))
+    free = int(round(free_last / dt))
+    zeros = np.zeros((system.getNumParticles(), 4), np.float32)
+    k = 0
+    while k < total:
+        m = min(every, total - k)
+        system.run(m, use_graph=True)
+        k += m
+        if k < total - free:
+            system.setArray(L.ARRAY_VELOCITY, zeros)
+            system.setArray(L.ARRAY_VELOCITYLEAPFROG, zeros)
+    system.set_params(P0)
+
+
+def release_gate(system, sc: Scenario) -> None:
+    """open the gate: move gate particles and clamp to the tank's far wall"""
+    for _ in range(sc.meta["release_shifts"]):
+        system.changeRightBoundary()
 
 
 def dambreak3d_particles(target_fluid: int, **kw) -> Scenario:

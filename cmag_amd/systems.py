@@ -337,3 +337,16 @@ class ParamSystem(SystemBase):
         meas[:, 0] = params.rest_density
         s.setArray(L.ARRAY_MEASURES, meas if measures4 is None else measures4)
         return s
+
+    # the scenario controls of the reference classes, for parameter-block systems
+    def changeRightBoundary(self) -> None:
+        """DamBreak: gate and clamp move by the column width (fluidSystem.cpp:239-255)"""
+        call("cmag_system_change_right_boundary", self._h)
+
+    def removeRightBoundary(self) -> None:
+        """DamBreak: gate parked, clamp lifted, clock restarted (fluidSystem.cpp:222-237)"""
+        call("cmag_system_remove_right_boundary", self._h)
+
+    def changeGravity(self) -> None:
+        """flip gravity's y component (poiseuilleFlowSystem.cpp:207-210)"""
+        call("cmag_system_change_gravity", self._h)

@@ -20,9 +20,12 @@ _lib = None
 
 
 def build() -> Path:
-    proc = subprocess.run(["make", "-s", "-C", str(HERE)], capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
+    """liboracle.so (AVX2, runs on any x86-64-v3 host) and liboracle_v4.so
+    (AVX-512, the CPU baseline's build on hosts that have it)"""
+    for extra in ([], ["OUT=liboracle_v4.so", "MARCH=x86-64-v4"]):
+        proc = subprocess.run(["make", "-s", "-C", str(HERE)] + extra, capture_output=True, text=True)
+        if proc.returncode != 0:
+            raise RuntimeError(f"oracle build failed:\n{proc.stdout}\n{proc.stderr}")
     return LIB_PATH
 
 
@@ -42,6 +45,21 @@ def lib() -> C.CDLL:
             build()
         _lib = _load(LIB_PATH)
     return _lib
+
+
+def host_has_avx512() -> bool:
+    try:
+        flags = Path("/proc/cpuinfo").read_text()
+    except OSError:
+        return False
+    return all(f" {f}" in flags for f in ("avx512f", "avx512bw", "avx512dq", "avx512vl"))
+
+
+def use_default(path) -> None:
+    """load the library at path (another ISA build of the same source) for
+    every later call"""
+    global _lib
+    _lib = _load(Path(path))
 
 
 def build_variant(mutation: int, outdir) -> Path:

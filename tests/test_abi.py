@@ -133,7 +133,20 @@ def test_scenarios_geometry():
                          - p.world_origin[0] - 0.5 / 64)) <= 1.01e-3 / 64
     db = S.dambreak3d(dx=0.02)
     assert db.num_fluid == 20 * 40 * 20
-    assert np.all(db.pos[:db.num_fluid, 3] == 4.0) and np.all(db.pos[db.num_fluid:, 3] == 1.0)
+    assert np.all(db.pos[:db.num_fluid, 3] == 4.0)
+    w = db.pos[db.num_fluid:]
+    assert np.all((w[:, 3] == 1.0) | (w[:, 3] == 0.0))
+    # the gate (RightFirst walls) fills the tank's cross-section at the column's face,
+    # three layers thick, and the release shifts put it on the tank's far wall
+    gate = w[w[:, 3] == 0.0]
+    assert gate.shape[0] == 3 * 50 * 20 and db.meta["release_shifts"] == 3
+    assert np.allclose(np.unique(gate[:, 0]), 0.4 + (np.arange(3) + 0.5) * 0.02)
+    assert db.params.right_boundary == np.float32(0.4)
+    moved = gate[:, :3] + np.array([3 * 0.4, 0.0, 0.0], np.float32)
+    open_tank = S.dambreak3d(dx=0.02, gate=False)
+    fixed = open_tank.pos[open_tank.num_fluid:, :3]
+    key = lambda a: set(map(tuple, np.round(a / 0.001).astype(np.int64)))  # noqa: E731
+    assert key(moved) <= key(fixed) and len(key(np.concatenate([w[w[:, 3] == 1.0, :3], moved]))) == fixed.shape[0]
     assert np.unique(np.round(db.pos[:, :3] / 0.001).astype(np.int64), axis=0).shape[0] == db.pos.shape[0]
     big = S.dambreak3d(dx=5e-3)
     assert tuple(big.params.grid_size) == (128, 128, 64) and big.num_fluid == 1_024_000

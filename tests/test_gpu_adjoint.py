@@ -127,3 +127,79 @@ def test_gradient_independent_of_checkpoint_spacing(device, adjoint_problem):
     b = gpu_gradient(d, checkpoint_every=7)
     c = gpu_gradient(d, checkpoint_every=60)
     assert a == b == c
+
+
+def test_position_loss_gradient_matches_differences(device, oracle, adjoint_problem):
+    """A loss on the final positions, L = mean over the fluid of
+    sin(2 pi x / period) (continuous across the periodic wrap): its gradient
+    reaches mu and g only through the final-position adjoint, and matches
+    FP64 central differences of the oracle's forward run (1e-3 relative)."""
+    import torch
+
+    from cmag_amd.adjoint import ChannelRollout
+    d = adjoint_problem
+    sc, P = d["scenario"], d["params0"]
+    steps = 60
+    period = float(P.world_size[0])
+    run = ChannelRollout(P, sc.pos, sc.vel, steps=steps, checkpoint_every=20)
+    mu = torch.tensor(float(P.mu), dtype=torch.float64, requires_grad=True)
+    g = torch.tensor([float(P.gravity[a]) for a in range(3)], dtype=torch.float64, requires_grad=True)
+    pos, vel = run(mu, g)
+    fl = pos[:, 3] == 0.0
+    loss = torch.sin(2.0 * np.pi * pos[fl, 0].to(torch.float64) / period).mean()
+    loss.backward()
+    dmu, dg = float(mu.grad), float(g.grad[0])
+
+    def fp64_loss(mu=None, gx=None):
+        Q = P.copy()
+        if mu is not None:
+            Q.mu = mu
+        if gx is not None:
+            Q.gravity[0] = gx
+        s, _ = oracle.rollout(Q, sc.pos, sc.vel, sc.vel, steps)
+        f = s.pos[:, 3] == 0.0
+        return float(np.mean(np.sin(2.0 * np.pi * s.pos[f, 0] / period)))
+
+    eps = 1e-4
+    hi, lo = np.float32(float(P.mu) * (1 + eps)), np.float32(float(P.mu) * (1 - eps))
+    fd_mu = (fp64_loss(mu=hi) - fp64_loss(mu=lo)) / (float(hi) - float(lo))
+    g0 = float(P.gravity[0])
+    hi, lo = np.float32(g0 * (1 + eps)), np.float32(g0 * (1 - eps))
+    fd_g = (fp64_loss(gx=hi) - fp64_loss(gx=lo)) / (float(hi) - float(lo))
+    print(f"position loss: dL/dmu {dmu:.6e} vs {fd_mu:.6e}; dL/dg {dg:.6e} vs {fd_g:.6e}")
+    assert abs(dmu - fd_mu) <= 1e-3 * abs(fd_mu)
+    assert abs(dg - fd_g) <= 1e-3 * abs(fd_g)
+
+
+def test_two_calls_backpropagate_through_their_own_runs(device, adjoint_problem):
+    """Two calls of one rollout before backward (two viscosities in one loss):
+    each call's gradient equals the gradient of that call made alone."""
+    import torch
+
+    from cmag_amd.adjoint import ChannelRollout, profile_loss
+    d = adjoint_problem
+    sc, P = d["scenario"], d["params0"]
+    run = ChannelRollout(P, sc.pos, sc.vel, steps=40, checkpoint_every=20)
+    g0 = [float(P.gravity[a]) for a in range(3)]
+    mus = (float(P.mu), 1.5 * float(P.mu))
+
+    def alone(m):
+        mu = torch.tensor(m, dtype=torch.float64, requires_grad=True)
+        g = torch.tensor(g0, dtype=torch.float64, requires_grad=True)
+        pos, vel = run(mu, g)
+        profile_loss(pos, vel, P, d["target"], d["bins"]).backward()
+        return float(mu.grad), g.grad.clone()
+
+    a1, ga1 = alone(mus[0])
+    a2, ga2 = alone(mus[1])
+    mu1 = torch.tensor(mus[0], dtype=torch.float64, requires_grad=True)
+    mu2 = torch.tensor(mus[1], dtype=torch.float64, requires_grad=True)
+    g = torch.tensor(g0, dtype=torch.float64, requires_grad=True)
+    p1, v1 = run(mu1, g)
+    p2, v2 = run(mu2, g)
+    loss = profile_loss(p1, v1, P, d["target"], d["bins"]) + 2.0 * profile_loss(p2, v2, P, d["target"], d["bins"])
+    loss.backward()
+    assert float(mu1.grad) == a1
+    assert float(mu2.grad) == 2.0 * a2
+    assert torch.allclose(g.grad, ga1 + 2.0 * ga2, rtol=1e-12, atol=0.0)
+    assert a1 != a2

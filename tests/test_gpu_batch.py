@@ -163,3 +163,47 @@ def test_batch_errors(device):
         L.call("cmag_batch_get_array", b._h, 5, L.ARRAY_POSITION, None, 0)
     with pytest.raises(L.CmagError, match="block"):
         b.set_tuning(block=96)
+
+
+def test_channel_batch_256_full_workload(device, oracle):
+    """The benchmarked batch itself: 256 channels (128 Poiseuille, 128
+    peristaltic, 4.13M particles, so the global sort runs its large-tile path)
+    at the benchmark's tuned launch shape.  Every instance equals its single
+    System bit for bit after 20 graph-replayed steps, and after one step eight
+    instances (both kinds, the smallest and the largest) equal the FP32 oracle's
+    update() to 1e-5 (velocities, max-abs over max-abs) and 1e-6 (positions)."""
+    from tests.gpu_util import normwise
+    scen = S.channel_batch()
+    inst = [(sc.params, sc.pos, sc.vel) for sc in scen]
+    sizes = np.array([sc.pos.shape[0] for sc in scen])
+    print(f"channel batch: {len(scen)} instances, {sizes.sum():,} particles", flush=True)
+    assert len(scen) == 256 and sizes.sum() > (1 << 18)
+    b = Batch(inst)
+    b.set_tuning(block=256, nb_cap=8, blocks_per_cu=0, steps_per_graph=10)
+    # one eager step against the oracle on a sample of instances
+    b.update()
+    po, pe = np.arange(128), np.arange(128, 256)
+    pick = {int(po[np.argmin(sizes[po])]), int(po[np.argmax(sizes[po])]),
+            int(pe[np.argmin(sizes[pe])]), int(pe[np.argmax(sizes[pe])])}
+    rng = np.random.default_rng(4)
+    pick |= {int(k) for k in rng.choice(po, 2, replace=False)} | {int(k) for k in rng.choice(pe, 2, replace=False)}
+    for k in sorted(pick):
+        p, pos, vel = inst[k]
+        sim = oracle.Simulation(p, pos, vel, vel)
+        sim.update(1)
+        gp, gv = b.getArray(k, L.ARRAY_POSITION), b.getArray(k, L.ARRAY_VELOCITY)
+        ep = normwise(gp[:, :3], sim.pos[:, :3])
+        ev = normwise(gv[:, :3], sim.vel[:, :3])
+        print(f"instance {k} ({sizes[k]} particles, {'Poiseuille' if k < 128 else 'peristaltic'}): "
+              f"pos {ep:.2e}, vel {ev:.2e}", flush=True)
+        assert ep <= 1e-6 and ev <= 1e-5
+    # 20 more steps in two graphs of 10, against the single Systems
+    ss = singles(inst)
+    b.run(20, use_graph=True)
+    for s in ss:
+        for _ in range(21):
+            s.update()
+    assert_same(b, ss)
+    for s in ss:
+        s.close()
+    b.close()

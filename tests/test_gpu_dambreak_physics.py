@@ -1,0 +1,83 @@
+"""The 3-D dam break as a dam break (a physical check, not a parity one): the
+1M-fluid column (dx = 5 mm) settles behind its gate, then runs out along the
+tank floor as Martin & Moyce's column collapse does.  Scenario controls the
+reference's report harnesses use (changeRightBoundary, changeGravity) are
+checked on the way."""
+import math
+
+import numpy as np
+import pytest
+
+from cmag_amd import _lib as L
+from cmag_amd import scenarios as S
+from cmag_amd.systems import ParamSystem
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dambreak3d_1m_settles_then_follows_martin_moyce(device):
+    """Relaxation: 0.3 s behind the gate with alpha = 2 and kinetic damping,
+    the last 0.02 s undamped; the fastest fluid particle is then below 0.1 c0.
+    Release (the gate moved to the far wall), with the reference's artificial
+    viscosity alpha = 0.38 (fluid_kernel.cu:233): the surge front
+    x_front / a at Martin & Moyce's times T = t sqrt(2g/a) stays within 20 %
+    of the experiment (the bound of the 2-D report harness check).  At the
+    benchmark's alpha = 0.01 the front runs ahead with spray (printed, not
+    asserted)."""
+    sc = S.dambreak3d(dx=5e-3)
+    nf, c0, a = sc.num_fluid, sc.meta["c0"], sc.meta["column"][0]
+    print(f"1M gated dam break: {nf} fluid + {sc.meta['num_wall']} wall particles, c0 {c0:.2f}", flush=True)
+    results = {}
+    for alpha in (0.38, 0.01):
+        P = sc.params.copy()
+        P.alpha = alpha
+        s = ParamSystem.from_state(P, sc.pos, sc.vel)
+        S.relax_gated(s, sc)
+        v = np.linalg.norm(s.getArray(L.ARRAY_VELOCITY)[:nf, :3], axis=1)
+        print(f"alpha {alpha}: relaxed, vmax {v.max():.3f} m/s, v99 {np.quantile(v, 0.99):.3f}, "
+              f"median {np.median(v):.4f}", flush=True)
+        assert v.max() < 0.1 * c0
+        S.release_gate(s, sc)
+        assert s.params.right_boundary == pytest.approx(sc.meta["tank"][0], rel=1e-6)
+        dt = float(P.delta_time)
+        scale = math.sqrt(2.0 * 9.81 / a)
+        k, rows = 0, []
+        for T, z_exp in S.MARTIN_MOYCE_SURGE:
+            m = int(math.ceil(T / (scale * dt))) - k
+            if m > 0:
+                s.run(m, use_graph=True)
+                k += m
+            pos = s.getArray(L.ARRAY_POSITION, 0, nf)
+            vmax = float(np.linalg.norm(s.getArray(L.ARRAY_VELOCITY, 0, nf)[:, :3], axis=1).max())
+            z = (float(pos[:, 0].max()) + 0.5 * sc.meta["dx"]) / a
+            rows.append((T, z_exp, z, vmax))
+            print(f"alpha {alpha}: T {T:.2f} Z_exp {z_exp:.2f} Z {z:.3f} ({(z - z_exp) / z_exp:+.3f}) "
+                  f"vmax {vmax:.2f}", flush=True)
+        results[alpha] = np.array(rows)
+        assert np.all(np.isfinite(pos))
+        s.close()
+    r = results[0.38]
+    assert np.all(np.abs(r[:, 2] - r[:, 1]) / r[:, 1] < 0.2)
+    assert np.all(np.diff(r[:, 2]) > 0)
+
+
+def test_change_gravity_matches_oracle(device, oracle):
+    """changeGravity flips gravity's y component (poiseuilleFlowSystem.cpp:
+    207-210): steps after the flip equal the oracle's with the flipped
+    parameter block."""
+    sc = S.poiseuille_channel()
+    P = sc.params.copy()
+    P.gravity[1] = np.float32(3e-4)
+    s = ParamSystem.from_state(P, sc.pos, sc.vel)
+    s.changeGravity()
+    assert s.params.gravity[1] == np.float32(-3e-4)
+    Q = P.copy()
+    Q.gravity[1] = np.float32(-3e-4)
+    sim = oracle.Simulation(Q, sc.pos, sc.vel, sc.vel)
+    s.run(3, use_graph=False)
+    sim.update(3)
+    assert np.array_equal(s.getUintArray(L.ARRAY_INDEX), sim.index)
+    v = s.getArray(L.ARRAY_VELOCITY)
+    from tests.gpu_util import normwise
+    assert normwise(v[:, :2], sim.vel[:, :2]) <= 1e-5
+    s.close()
