@@ -171,6 +171,8 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_system_set_stream": (C.c_int, [_vp, _vp]),
     "cmag_system_set_subcell_walk": (C.c_int, [_vp, C.c_int]),
     "cmag_system_subcell_walk": (C.c_int, [_vp]),
+    "cmag_system_set_inrange_masks": (C.c_int, [_vp, C.c_int]),
+    "cmag_system_inrange_masks": (C.c_int, [_vp]),
     "cmag_system_synchronize": (C.c_int, [_vp]),
     "cmag_system_enable_timing": (C.c_int, [_vp, C.c_int]),
     "cmag_system_stage_times": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]),

@@ -57,11 +57,14 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
                          const float* sortedVel, const uint32_t* cellFirst, uint32_t n, uint32_t numCells,
                          uint32_t* pre, const uint8_t* bucket, float* posW, float* velW, uint32_t* indexW,
                          uint32_t* w2r, hipStream_t s);
+// masks (optional, 2 * MASK_ROWS words per particle): density leaves the
+// in-range masks of staged blocks there and force reads them
 int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2r, float* posW, float* velW,
-                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s);
+                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s,
+                       uint32_t* masks = nullptr);
 int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const float* velW,
                      const uint32_t* indexW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
-                     hipStream_t s);
+                     hipStream_t s, const uint32_t* masks = nullptr);
 int launch_force(const cmag_params& P, float* acc, const float* measures, const float* sortedPos,
                  const float* sortedVel, const uint32_t* index, const uint32_t* cellStart,
                  const uint32_t* cellEnd, uint32_t n, hipStream_t s,

@@ -222,10 +222,13 @@ int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint3
 // force pass reads: -rho in posW[i].w and p / rho^2 in velW[i].w.  Other threads read
 // posW rows during this pass, but only their x, y, z; a thread writes the w of
 // its own row only, after its walk.
+// With masks, staged blocks also leave the in-range masks of their rows for
+// the force pass (masks[r * n + i]).
 template <int DIM, int HASH>
 __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, float4* __restrict__ measures,
                                                            const uint32_t* __restrict__ w2r, float4* posW,
-                                                           float4* velW, const SubCells cells, uint32_t n) {
+                                                           float4* velW, const SubCells cells, uint32_t n,
+                                                           uint2* __restrict__ masks) {
     __shared__ LookupPlan lp;
     __shared__ LookupTables lt;
     const uint32_t b0 = xcd_block() * blockDim.x;
@@ -236,8 +239,9 @@ __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, f
     if (staged) stage_rows<DIM>(lp, lt, posW, rows, rs);
     if (i >= n) return;
     const float4 pi = posW[i];
-    const float dens = staged ? density_sum_lookup<DIM>(P, pi, lp, lt, rs, &rows, posW)
-                              : density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells);
+    const float dens = !staged ? density_sum<CMAG_SYS_DAMBREAK, DIM, HASH>(P, pi, posW, cells)
+                       : masks ? density_sum_masks<DIM>(P, pi, lp, lt, rs, &rows, posW, masks, i, n)
+                               : density_sum_lookup<DIM>(P, pi, lp, lt, rs, &rows, posW);
     const float p = eos(P, dens);
     reinterpret_cast<float2*>(measures)[2 * (size_t)w2r[i]] = make_float2(dens, p);
     if (pi.w == CMAG_DB_FLUID) {
@@ -247,21 +251,23 @@ __global__ __launch_bounds__(256) void density_sub_kernel(const cmag_params P, f
 }
 
 int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2r, float* posW, float* velW,
-                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s) {
+                       const uint32_t* cellFirst, const uint32_t* pre, uint32_t n, hipStream_t s,
+                       uint32_t* masks) {
     if (n == 0) return CMAG_OK;
     if (P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_INVALID, "the sub-cell walk is DamBreak's");
     const SubCells cells{cellFirst, reinterpret_cast<const uint4*>(pre)};
     float4* m4 = reinterpret_cast<float4*>(measures);
     float4* pw = reinterpret_cast<float4*>(posW);
     float4* vw = reinterpret_cast<float4*>(velW);
+    uint2* mk = reinterpret_cast<uint2*>(masks);
     dim3 g(div_up(n, 256)), b(256);
     const bool m = P.hash_order == CMAG_HASH_MORTON;
     if (P.dim == 3) {
-        if (m) hipLaunchKernelGGL((density_sub_kernel<3, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
-        else hipLaunchKernelGGL((density_sub_kernel<3, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
+        if (m) hipLaunchKernelGGL((density_sub_kernel<3, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n, mk);
+        else hipLaunchKernelGGL((density_sub_kernel<3, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n, mk);
     } else {
-        if (m) hipLaunchKernelGGL((density_sub_kernel<2, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
-        else hipLaunchKernelGGL((density_sub_kernel<2, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n);
+        if (m) hipLaunchKernelGGL((density_sub_kernel<2, 1>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n, mk);
+        else hipLaunchKernelGGL((density_sub_kernel<2, 0>), g, b, 0, s, P, m4, w2r, pw, vw, cells, n, mk);
     }
     return check_launch("density_sub_kernel");
 }
@@ -274,7 +280,7 @@ __global__ __launch_bounds__(256) void force_sub_kernel(const cmag_params P, flo
                                                          const float4* __restrict__ posW,
                                                          const float4* __restrict__ velW,
                                                          const uint32_t* __restrict__ indexW, const SubCells cells,
-                                                         uint32_t n) {
+                                                         uint32_t n, const uint2* __restrict__ masks) {
     __shared__ uint32_t nbl[256 / 64][NB_CAP][64];
     __shared__ LookupPlan lp;
     __shared__ LookupTables lt;
@@ -283,12 +289,12 @@ __global__ __launch_bounds__(256) void force_sub_kernel(const cmag_params P, flo
     const bool staged = stage_lookups<DIM>(P, posW, cells, n, b0, lp, lt);
     if (i >= n) return;
     force_body<CMAG_SYS_DAMBREAK, DIM, HASH>(P, i, acc, nullptr, posW, velW, indexW, cells, n, nbl[threadIdx.x >> 6],
-                                             staged ? &lp : nullptr, &lt);
+                                             staged ? &lp : nullptr, &lt, masks, n);
 }
 
 int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const float* velW,
                      const uint32_t* indexW, const uint32_t* cellFirst, const uint32_t* pre, uint32_t n,
-                     hipStream_t s) {
+                     hipStream_t s, const uint32_t* masks) {
     if (n == 0) return CMAG_OK;
     if (n >= (1u << NB_SHIFT_BITS)) return fail(CMAG_E_INVALID, "force passes take at most 2^27 particles");
     if (P.system != CMAG_SYS_DAMBREAK) return fail(CMAG_E_INVALID, "the sub-cell walk is DamBreak's");
@@ -297,7 +303,8 @@ int launch_force_sub(const cmag_params& P, float* acc, const float* posW, const 
         hipLaunchKernelGGL((force_sub_kernel<decltype(dim)::value, decltype(hash)::value>),
                            dim3(div_up(n, 256)), dim3(256), 0, s, P, reinterpret_cast<float4*>(acc),
                            reinterpret_cast<const float4*>(posW),
-                           reinterpret_cast<const float4*>(velW), indexW, cells, n);
+                           reinterpret_cast<const float4*>(velW), indexW, cells, n,
+                           reinterpret_cast<const uint2*>(masks));
     };
     const bool m = P.hash_order == CMAG_HASH_MORTON;
     using I2 = std::integral_constant<int, 2>;

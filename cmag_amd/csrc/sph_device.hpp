@@ -485,7 +485,112 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
     }
 }
 
-constexpr int NB_CAP = 16;
+// ---- in-range masks (staged blocks of the sub-cell walk) ---------------------
+// The density pass tests every candidate of a staged block anyway; it leaves,
+// per target and stencil row r, a 64-bit mask whose bit t says that candidate
+// s + t of the row's main run [s, e) lies within 2h (r^2 < 4h^2, the force
+// pass's own test on the same float operations).  The force pass then pushes a
+// wall-free row's in-range neighbours straight from the mask, in the walk's
+// order, without loading or testing its candidates.  Masks are stored
+// row-major, [r][target] (uint2: low word bits 0..31); a row whose run is
+// longer than MASK_BITS or holds walls is walked as before.
+constexpr int MASK_BITS = 64;
+constexpr int MASK_ROWS = 9;
+
+// the in-range bits of a candidate pair (t, t + 1) of a run, placed at t
+// (t is even); bits past MASK_BITS wrap and are never read
+__device__ __forceinline__ uint64_t mask_pair(uint32_t t, float r2a, float r2b, float r2max) {
+    const uint32_t b = (r2a < r2max ? 1u : 0u) | (r2b < r2max ? 2u : 0u);
+    return (uint64_t)b << (t & 63u);
+}
+
+// density of the target pi over a staged block's lookups, as
+// density_sum_lookup, writing the in-range masks of its rows' main runs to
+// masks[r * stride + i]
+template <int DIM>
+__device__ __forceinline__ float density_sum_masks(const cmag_params& P, const float4& pi, const LookupPlan& lp,
+                                                   const LookupTables& lt, const RowStage& rs, const RowLds* lds,
+                                                   const float4* __restrict__ pos, uint2* __restrict__ masks,
+                                                   uint32_t i, uint32_t stride) {
+    constexpr int R = DIM == 3 ? 9 : 3;
+    const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
+    const int q = x_quarter(P, pi.x);
+    const uint32_t dx = (uint32_t)((gp.x & ((int)P.grid_size[0] - 1)) - lp.cx0);
+    const float half_inv_h = -0.5f / P.smoothing_radius;
+    const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
+    float sum = 0.0f;
+    auto term = [&](float r2) {
+        const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
+        const float t2 = t * t;
+        sum = fmaf(t2 * t2, fmaf(-4.0f, t, 5.0f), sum);
+    };
+    auto d2 = [&](const float4& pj) { return norm2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z); };
+    auto cand = [&](uint32_t, const float4& pj, bool valid, float, float, float, int) {
+        term(valid ? d2(pj) : 1e30f);
+    };
+    for (int r = 0; r < R; ++r) {
+        uint32_t s = lt.first[r][dx];
+        if (q > 0) s += SubCells::through(lt.pre[r][dx], q - 1);
+        const uint32_t f1 = lt.first[r][dx + 2];
+        const uint4 c1 = lt.pre[r][dx + 2];
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt.first[r][dx + 3];
+        uint64_t m = 0;
+        const bool in_lds = DIM == 3 && rs.ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS;
+        if (in_lds) {
+            const int32_t off = rs.off[r - ROW_STAGE_FIRST];
+            for (uint32_t j = s; j < e; j += 2u) {
+                const int32_t l = (int32_t)j + off;  // in [0, staged rows]
+                const float2 a0 = lds->xy[l], a1 = lds->xy[l + 1];
+                const float ra = d2(make_float4(a0.x, a0.y, lds->z[l], 0.0f));
+                const float rb = j + 1u < e ? d2(make_float4(a1.x, a1.y, lds->z[l + 1], 0.0f)) : 1e30f;
+                term(ra);
+                term(rb);
+                m |= mask_pair(j - s, ra, rb, r2max);
+            }
+        } else {
+            // as walk_run_sub: two rows per iteration, each load one candidate ahead
+            const char* base = reinterpret_cast<const char*>(pos);
+            auto ld = [&](uint32_t off) { return *reinterpret_cast<const float4*>(base + off); };
+            const uint32_t end = e << 4, end1 = end - 16u;
+            uint32_t off = s << 4;
+            if (off < end) {
+                float4 p0 = ld(off);
+                do {
+                    const float4 p1 = ld(off + 16u);
+                    const float ra = d2(p0);
+                    term(ra);
+                    p0 = ld(off + 32u);
+                    const float rb = off < end1 ? d2(p1) : 1e30f;
+                    term(rb);
+                    m |= mask_pair((off >> 4) - s, ra, rb, r2max);
+                    off += 32u;
+                } while (off < end);
+            }
+        }
+        masks[(size_t)r * stride + i] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+        if (in_lds) {
+            const int32_t off = rs.off[r - ROW_STAGE_FIRST];
+            for (uint32_t j = ws; j < we; j += 2u) {
+                const int32_t l = (int32_t)j + off;
+                const float2 a0 = lds->xy[l], a1 = lds->xy[l + 1];
+                cand(j, make_float4(a0.x, a0.y, lds->z[l], 0.0f), true, 0.f, 0.f, 0.f, 13);
+                cand(j + 1u, make_float4(a1.x, a1.y, lds->z[l + 1], 0.0f), j + 1u < we, 0.f, 0.f, 0.f, 13);
+            }
+        } else {
+            walk_run_sub(pos, ws, we, pi.x, pi.y, pi.z, 13, cand);
+        }
+    }
+    return sum * P.kernel_coeff * P.particle_mass;
+}
+
+// queue depth and drain size (compile-time; -D overrides for tuning builds)
+#ifndef CMAG_NB_CAP
+#define CMAG_NB_CAP 16
+#endif
+#ifndef CMAG_NB_DRAIN
+#define CMAG_NB_DRAIN 2
+#endif
+constexpr int NB_CAP = CMAG_NB_CAP;
 constexpr int NB_SHIFT_BITS = 27;  // entry = j | (image-shift code << 27)
 
 struct NbEntry {
@@ -503,7 +608,7 @@ struct NbEntry {
 // NB_DRAIN oldest entries (fewer if it has fewer), which keeps all lanes busy
 // in the pair loop instead of waiting for the fullest one; a final drain
 // empties the queues.
-constexpr int NB_DRAIN = 2;
+constexpr int NB_DRAIN = CMAG_NB_DRAIN;
 
 template <int CAP>
 struct NbQueue {
@@ -723,6 +828,10 @@ struct ForceSum {
 // density pass's results: a fluid row holds -rho in pos.w, sign bit set (a wall
 // row keeps its type flag, 0..3, sign bit clear) and p / rho^2 in vel.w, so a
 // pair term gathers two rows instead of three and measures is not read.
+// With masks (staged blocks of the sub-cell walk), the in-range neighbours of a
+// row whose main run holds no walls and at most MASK_BITS candidates come from
+// the density pass's mask instead of a walk; they are pushed in the walk's
+// order, so the sums are the same bit for bit.
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -730,7 +839,8 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                                            const float4* __restrict__ sortedVel,
                                            const uint32_t* __restrict__ index, const Cells& cells, uint32_t n,
                                            uint32_t (*nbl)[64], const LookupPlan* lp = nullptr,
-                                           const LookupTables* lt = nullptr) {
+                                           const LookupTables* lt = nullptr,
+                                           const uint2* __restrict__ masks = nullptr, uint32_t stride = 0) {
     const int lane = threadIdx.x & 63;
     constexpr bool PACKED = SYS == CMAG_SYS_DAMBREAK && Cells::SUB;
     auto is_wall = [](float w) { return PACKED ? __float_as_int(w) >= 0 : w != CMAG_DB_FLUID; };
@@ -786,8 +896,38 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
     };
     if constexpr (Cells::SUB) {
-        if (lp) walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), *lp, *lt, sortedPos, cand);
-        else walk_stencil_pf<DIM, HASH, false>(P, pi, gp, cells, sortedPos, cand);
+        if (lp && masks) {
+            constexpr int R = DIM == 3 ? 9 : 3;
+            const int q = x_quarter(P, pi.x);
+            const uint32_t dx = (uint32_t)((gp.x & ((int)P.grid_size[0] - 1)) - lp->cx0);
+            for (int r = 0; r < R; ++r) {
+                uint32_t s = lt->first[r][dx];
+                const uint4 c0 = lt->pre[r][dx];
+                if (q > 0) s += SubCells::through(c0, q - 1);
+                const uint32_t f1 = lt->first[r][dx + 2];
+                const uint4 c1 = lt->pre[r][dx + 2];
+                const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt->first[r][dx + 3];
+                // walls of the run's first two cells (the last cell's are [ws, we))
+                const uint32_t walls = (f1 - lt->first[r][dx]) - c0.w - lt->pre[r][dx + 1].w;
+                if (walls != 0u || e - s > (uint32_t)MASK_BITS) {
+                    walk_run_sub(sortedPos, s, e, pi.x, pi.y, pi.z, 13, cand);
+                } else {
+                    const uint2 mw = masks[(size_t)r * stride + i];
+                    uint64_t m = ((uint64_t)mw.y << 32) | mw.x;
+                    while (m != 0u) {
+                        const uint32_t j = s + (uint32_t)__builtin_ctzll(m);
+                        m &= m - 1u;
+                        if (j != i) nq.push(NbEntry::pack(j, 13));
+                        if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
+                    }
+                }
+                walk_run_sub(sortedPos, ws, we, pi.x, pi.y, pi.z, 13, cand);
+            }
+        } else if (lp) {
+            walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), *lp, *lt, sortedPos, cand);
+        } else {
+            walk_stencil_pf<DIM, HASH, false>(P, pi, gp, cells, sortedPos, cand);
+        }
     } else {
         walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos, cand);
     }

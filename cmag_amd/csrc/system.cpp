@@ -44,6 +44,11 @@ struct cmag_system {
     float *posW = nullptr, *velW = nullptr;
     uint32_t *indexW = nullptr, *w2r = nullptr, *preW = nullptr;
     uint8_t* bucketR = nullptr;
+    // in-range masks the density pass leaves for the force pass (staged blocks
+    // of the sub-cell walk): 9 rows x 64 bits per particle
+    static constexpr size_t MASK_WORDS = 18;
+    bool use_masks = true;
+    uint32_t* masks = nullptr;
 
     float elapsed = 0.0f;
     // Peristalsis boundary-configuration state (PeristalsisSystem.cpp:330-343):
@@ -89,7 +94,8 @@ struct cmag_system {
             if (e) (void)hipEventDestroy(e);
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
-                        gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, indexW, w2r, preW, bucketR};
+                        gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, indexW, w2r, preW, bucketR,
+                        masks};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -150,6 +156,7 @@ struct cmag_system {
             // padded to whole 16-byte words: the move kernel reads buckets 16 at a time
             CMAG_HIP_TRY(hipMalloc((void**)&bucketR, ((size_t)n + 31) & ~(size_t)15));
             CMAG_HIP_TRY(hipMalloc((void**)&preW, sizeof(uint32_t) * 4 * (size_t)numCells));
+            CMAG_HIP_TRY(hipMalloc((void**)&masks, sizeof(uint32_t) * MASK_WORDS * (size_t)(n ? n : 1)));
         }
         if (on != subwalk) invalidate_graph();
         subwalk = on;
@@ -186,7 +193,8 @@ struct cmag_system {
                                           posW, velW, indexW, w2r, stream));
         CMAG_TRY(mark(CMAG_STAGE_DENSITY));
         if (subwalk)
-            CMAG_TRY(launch_density_sub(P, measures, w2r, posW, velW, cellFirst, preW, n, stream));
+            CMAG_TRY(launch_density_sub(P, measures, w2r, posW, velW, cellFirst, preW, n, stream,
+                                        use_masks ? masks : nullptr));
         else
             CMAG_TRY(launch_density(P, measures, sortedPos, cellStart, cellEnd, n, stream, cellFirst));
         return CMAG_OK;
@@ -200,7 +208,8 @@ struct cmag_system {
             CMAG_TRY(launch_peristalsis_forces(P, acc, presF, measures, sortedPos, sortedVel, index,
                                                cellStart, cellEnd, n, stream, cellFirst));
         } else if (subwalk) {
-            CMAG_TRY(launch_force_sub(P, acc, posW, velW, indexW, cellFirst, preW, n, stream));
+            CMAG_TRY(launch_force_sub(P, acc, posW, velW, indexW, cellFirst, preW, n, stream,
+                                      use_masks ? masks : nullptr));
         } else {
             CMAG_TRY(launch_force(P, acc, measures, sortedPos, sortedVel, index, cellStart, cellEnd, n,
                                   stream, cellFirst));
@@ -607,6 +616,16 @@ int cmag_system_set_subcell_walk(cmag_system* s, int on) {
 }
 
 int cmag_system_subcell_walk(const cmag_system* s) { return s && s->subwalk ? 1 : 0; }
+
+int cmag_system_set_inrange_masks(cmag_system* s, int on) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    CMAG_TRY(s->collect());
+    if ((on != 0) != s->use_masks) s->invalidate_graph();
+    s->use_masks = on != 0;
+    return CMAG_OK;
+}
+
+int cmag_system_inrange_masks(const cmag_system* s) { return s && s->use_masks ? 1 : 0; }
 
 int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream) {
     if (!s) return fail(CMAG_E_INVALID, "null system");

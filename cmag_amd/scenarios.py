@@ -260,7 +260,7 @@ def poiseuille_series(y: np.ndarray, t: float, F: float, l: float, nu: float,
 def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), layers: int = 3,
                h_factor: float = 1.3, alpha: float = 0.01, jitter: float = 0.01, seed: int = 1973,
                grid=None, hash_order: int = L.HASH_LINEAR, cell_factor: float = 1.0,
-               morton_xbits: int = 0, gate: bool = True) -> Scenario:
+               morton_xbits: int = 0, gate: bool | None = None) -> Scenario:
     """3-D dam break in the DamBreak system's loop bodies (Tait EOS, Monaghan
     artificial viscosity, Lennard-Jones wall particles, clamp walls).
 
@@ -276,7 +276,9 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
     relax before it is released; changeRightBoundary moves gate and clamp by
     the column width (fluid_size.x * 2r) per call, and
     meta["release_shifts"] calls put both at the tank's far wall (release).
-    Without the gate the column starts released."""
+    Without the gate the column starts released.  gate=None: gated where the
+    lattice allows it (tank width a whole multiple of the column width in
+    lattice steps), else released."""
     h = h_factor * dx
     cs = cell_factor * 2.0 * h
     cellcount = int(math.ceil(2.0 * h / cs - 1e-9))
@@ -345,6 +347,12 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
     X, Y, Z = np.meshgrid(left_x, ys, zs, indexing="ij")
     walls.append(np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1))
     inner_z = (np.arange(0, nz_t) + 0.5) * dx
+    fits = nfx > 0 and nx_t % nfx == 0 and nx_t > nfx
+    if gate is None:
+        gate = fits
+    elif gate and not fits:
+        raise ValueError("the gate reaches the tank's far wall only if the tank is a whole multiple of the column "
+                         "(in lattice steps)")
     if gate:
         # the right wall's columns beside the tank's z walls stay put; its
         # inner part is the gate, at the column's face until released
@@ -363,9 +371,7 @@ def dambreak3d(dx: float = 5e-3, tank=(1.6, 1.0, 0.4), column=(0.4, 0.8, 0.4), l
     parts = [fluid, wall]
     shifts = 0
     if gate:
-        shifts = int(round((tank[0] - column[0]) / (nfx * dx)))
-        if abs(shifts * nfx * dx - (tank[0] - column[0])) > 1e-6 * tank[0]:
-            raise ValueError("the gate reaches the tank's far wall only if tank - column is a multiple of column")
+        shifts = nx_t // nfx - 1
         gx = right_x - shifts * nfx * dx
         X, Y, Z = np.meshgrid(gx, ys, inner_z, indexing="ij")
         g4 = np.empty((X.size, 4), np.float64)

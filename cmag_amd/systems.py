@@ -134,6 +134,15 @@ class SystemBase:
     def subcell_walk(self) -> bool:
         return bool(lib().cmag_system_subcell_walk(self._h))
 
+    def set_inrange_masks(self, on: bool) -> None:
+        """the force pass takes wall-free rows' in-range neighbours from the
+        density pass's masks (on by default; forces identical either way)"""
+        call("cmag_system_set_inrange_masks", self._h, 1 if on else 0)
+
+    @property
+    def inrange_masks(self) -> bool:
+        return bool(lib().cmag_system_inrange_masks(self._h))
+
     def synchronize(self) -> None:
         call("cmag_system_synchronize", self._h)
 

@@ -225,6 +225,14 @@ int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream);
  * CMAG_E_INVALID for others. */
 int cmag_system_set_subcell_walk(cmag_system* s, int on);
 int cmag_system_subcell_walk(const cmag_system* s);
+/* In-range masks of the sub-cell walk (no counterpart in the reference): with
+ * on != 0 (the default) the density pass leaves, for the blocks whose cell
+ * lookups it stages, a 64-bit in-range mask per target and stencil row, and
+ * the force pass takes the in-range neighbours of wall-free rows from it
+ * instead of testing the candidates again.  Forces are the same bit for bit
+ * either way; 72 bytes of device memory per particle. */
+int cmag_system_set_inrange_masks(cmag_system* s, int on);
+int cmag_system_inrange_masks(const cmag_system* s);
 int cmag_system_synchronize(cmag_system* s);
 
 /* stage timing (no reference counterpart): with timing on, update() records

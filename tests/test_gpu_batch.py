@@ -171,7 +171,11 @@ def test_channel_batch_256_full_workload(device, oracle):
     at the benchmark's tuned launch shape.  Every instance equals its single
     System bit for bit after 20 graph-replayed steps, and after one step eight
     instances (both kinds, the smallest and the largest) equal the FP32 oracle's
-    update() to 1e-5 (velocities, max-abs over max-abs) and 1e-6 (positions)."""
+    update(): densities to 1e-5 relative and positions to 1e-6.  Forces and
+    velocities one step from rest are nearly cancelling pressure terms (a
+    channel's jittered lattice), whose FP32 sums carry rounding of ~1e-4 of
+    the net field in any summation order: there the GPU must be as close to
+    the FP64 oracle as the FP32 oracle itself is (within 4x, or 1e-5)."""
     from tests.gpu_util import normwise
     scen = S.channel_batch()
     inst = [(sc.params, sc.pos, sc.vel) for sc in scen]
@@ -191,12 +195,23 @@ def test_channel_batch_256_full_workload(device, oracle):
         p, pos, vel = inst[k]
         sim = oracle.Simulation(p, pos, vel, vel)
         sim.update(1)
+        s64 = oracle.Simulation(p, pos, vel, vel, dtype=np.float64)
+        s64.update(1)
         gp, gv = b.getArray(k, L.ARRAY_POSITION), b.getArray(k, L.ARRAY_VELOCITY)
+        gm, ga = b.getArray(k, L.ARRAY_MEASURES), b.getArray(k, L.ARRAY_ACCELERATION)
+        fields = [(ga, sim.acc, s64.acc), (gv, sim.vel, s64.vel)]
+        if k >= 128:
+            fields.append((b.getArray(k, L.ARRAY_PRESSUREFORCE), sim.presF, s64.presF))
         ep = normwise(gp[:, :3], sim.pos[:, :3])
-        ev = normwise(gv[:, :3], sim.vel[:, :3])
+        ed = float(np.max(np.abs(gm[:, 0].astype(np.float64) - sim.measures[:, 0]) / sim.measures[:, 0]))
+        msg = []
+        for got, w32, w64 in fields:
+            e_gpu, e_32 = normwise(got[:, :3], w64[:, :3]), normwise(w32[:, :3], w64[:, :3])
+            msg.append(f"{e_gpu:.1e} (FP32 oracle {e_32:.1e})")
+            assert e_gpu <= max(1e-5, 4.0 * e_32), (k, e_gpu, e_32)
         print(f"instance {k} ({sizes[k]} particles, {'Poiseuille' if k < 128 else 'peristaltic'}): "
-              f"pos {ep:.2e}, vel {ev:.2e}", flush=True)
-        assert ep <= 1e-6 and ev <= 1e-5
+              f"density {ed:.2e}, pos {ep:.2e}; vs FP64: force/vel/pressure {', '.join(msg)}", flush=True)
+        assert ed <= 1e-5 and ep <= 1e-6
     # 20 more steps in two graphs of 10, against the single Systems
     ss = singles(inst)
     b.run(20, use_graph=True)

@@ -77,7 +77,12 @@ def test_change_gravity_matches_oracle(device, oracle):
     s.run(3, use_graph=False)
     sim.update(3)
     assert np.array_equal(s.getUintArray(L.ARRAY_INDEX), sim.index)
-    v = s.getArray(L.ARRAY_VELOCITY)
     from tests.gpu_util import normwise
-    assert normwise(v[:, :2], sim.vel[:, :2]) <= 1e-5
+    # velocities from rest are nearly cancelling pressure terms (1e-4, as the
+    # channel's own oracle checks); the y component carries the flipped gravity
+    v = s.getArray(L.ARRAY_VELOCITY)
+    assert normwise(v[:, :2], sim.vel[:, :2]) <= 1e-4
+    fl = sc.pos[:, 3] == 0.0
+    assert np.all(v[fl, 1] < 0.0) and np.all(sim.vel[fl, 1] < 0.0)
+    assert normwise(s.getArray(L.ARRAY_POSITION)[:, :2], sim.pos[:, :2]) <= 1e-6
     s.close()
