@@ -118,16 +118,23 @@ struct SubCells {
     static constexpr bool SUB = true;
     const uint32_t* __restrict__ first;
     const uint4* __restrict__ pre;
-    __device__ __forceinline__ static uint32_t through(const uint4& c, int q) {
-        return q == 0 ? c.x : q == 1 ? c.y : q == 2 ? c.z : c.w;
+    // one of four values by q (0..3) with selects rather than branches (the
+    // quarter differs between the lanes of a wave)
+    __device__ __forceinline__ static uint32_t pick4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int q) {
+        const uint32_t lo = (q & 1) ? b : a;
+        const uint32_t hi = (q & 1) ? d : c;
+        return (q & 2) ? hi : lo;
     }
+    // the fluid of quarters <= q, and of quarters < q
+    __device__ __forceinline__ static uint32_t through(const uint4& c, int q) { return pick4(c.x, c.y, c.z, c.w, q); }
+    __device__ __forceinline__ static uint32_t below(const uint4& c, int q) { return pick4(0u, c.x, c.y, c.z, q); }
     // the run k0..k1 trimmed for a target in quarter q: lo (k0 is the
     // stencil's first cell along x) drops k0's fluid quarters < q, hi (k1 is
     // the last) ends at k1's quarter q and returns k1's walls as [ws, we)
     __device__ __forceinline__ void range_sub(uint32_t k0, uint32_t k1, bool lo, bool hi, int q, uint32_t& s,
                                               uint32_t& e, uint32_t& ws, uint32_t& we) const {
         s = first[k0];
-        if (lo && q > 0) s += through(pre[k0], q - 1);
+        if (lo) s += below(pre[k0], q);
         ws = we = 0;
         if (hi) {
             const uint32_t f1 = first[k1];
@@ -341,26 +348,59 @@ __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const floa
 }
 
 // ---- cell lookups staged in LDS (sub-cell walk) -----------------------------
-// A block whose targets all lie in one row of cells along x, with the
-// stencil's x-range [cx0 - 1, cx1 + 1] inside the grid, on consecutive keys
-// and at most LK_W cells wide, copies the compact offsets and quarter counts
-// of those cells in its 9 (2-D: 3) stencil rows into LDS once; its targets
-// then find their runs there instead of with five loads per stencil row.
-// The runs and their order are the global walk's, so the sums are the same
-// bit for bit.
+// A block whose targets lie in one row of cells along x, or in two rows (the
+// end of one and the start of the next in key order), with each row's stencil
+// x-range [cx0 - 1, cx1 + 1] inside the grid, on consecutive keys and at most
+// LK_W cells wide (LK_W / 2 for each of two rows), copies the compact offsets
+// and quarter counts of those cells in the 9 (2-D: 3) stencil rows of each of
+// its rows into LDS once; its targets then find their runs there instead of
+// with five loads per stencil row.  The runs and their order are the global
+// walk's, so the sums are the same bit for bit.
 constexpr int LK_W = 32;
 
 struct LookupPlan {
-    uint32_t key[9];  // key of cell cx0 - 1 in each stencil row
-    uint32_t width;   // cells per stencil row segment
-    int cx0;
-    int ok;
+    uint32_t key[2][9];  // key of cell cx0 - 1 in each stencil row of each part
+    uint32_t width[2];   // cells per stencil row segment
+    int cx0[2];
+    int ry[2], rz[2];    // each part's row of cells (wrapped)
+    int parts;           // 0: not staged, 1 or 2 rows
+    int split;           // first target (block thread) of the second row
+    int bad;             // a target outside the two rows
 };
 
+// one part: rows r = 0..8 at first[r * (W + 1) + c], pre[r * W + c];
+// two parts: W = LK_W / 2, part 1's rows follow part 0's
 struct LookupTables {
-    uint32_t first[9][LK_W + 1];
-    uint4 pre[9][LK_W];
+    uint32_t first[18 * (LK_W / 2 + 1)];
+    uint4 pre[9 * LK_W];
 };
+
+// a target's view of the staged tables: entries from its cell x - 1 on, per
+// stencil row
+struct LookupView {
+    const uint32_t* first;
+    const uint4* pre;
+    uint32_t fs, ps;
+    __device__ __forceinline__ uint32_t F(int r, uint32_t c) const { return first[(uint32_t)r * fs + c]; }
+    __device__ __forceinline__ uint4 Pr(int r, uint32_t c) const { return pre[(uint32_t)r * ps + c]; }
+};
+
+template <int DIM>
+__device__ __forceinline__ LookupView lookup_view(const cmag_params& P, const LookupPlan& lp, const LookupTables& lt,
+                                                  const Cell& gp) {
+    constexpr int R = DIM == 3 ? 9 : 3;
+    const int wx = gp.x & ((int)P.grid_size[0] - 1), wy = gp.y & ((int)P.grid_size[1] - 1),
+              wz = gp.z & ((int)P.grid_size[2] - 1);
+    const int part = (lp.parts == 2 && wy == lp.ry[1] && wz == lp.rz[1]) ? 1 : 0;
+    const uint32_t W = lp.parts == 2 ? (uint32_t)(LK_W / 2) : (uint32_t)LK_W;
+    const uint32_t dx = (uint32_t)(wx - lp.cx0[part]);
+    LookupView v;
+    v.fs = W + 1u;
+    v.ps = W;
+    v.first = lt.first + (uint32_t)(part * R) * v.fs + dx;
+    v.pre = lt.pre + (uint32_t)(part * R) * v.ps + dx;
+    return v;
+}
 
 // all threads of the block call this; returns whether the block's lookups are staged
 template <int DIM>
@@ -368,36 +408,80 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
                                               const SubCells& cells, uint32_t n, uint32_t b0, LookupPlan& lp,
                                               LookupTables& lt) {
     constexpr int R = DIM == 3 ? 9 : 3;
+    const int gx = (int)P.grid_size[0], gy = (int)P.grid_size[1], gz = (int)P.grid_size[2];
+    const bool eligible = b0 < n && P.cellcount == 1;
+    const uint32_t b1 = eligible ? min(b0 + blockDim.x, n) - 1u : 0u;
+    auto wrapped = [&](uint32_t k) {
+        const float4 p = pos[k];
+        Cell c = grid_pos(P, p.x, p.y, p.z);
+        c.x &= gx - 1;
+        c.y &= gy - 1;
+        c.z &= gz - 1;
+        return c;
+    };
     if (threadIdx.x == 0) {
-        int ok = 0;
-        if (b0 < n && P.cellcount == 1) {
-            const uint32_t b1 = min(b0 + blockDim.x, n) - 1u;
-            const float4 pa = pos[b0], pb = pos[b1];
-            const Cell a = grid_pos(P, pa.x, pa.y, pa.z), b = grid_pos(P, pb.x, pb.y, pb.z);
-            const int gx = (int)P.grid_size[0], gy = (int)P.grid_size[1], gz = (int)P.grid_size[2];
-            const int ax = a.x & (gx - 1), ay = a.y & (gy - 1), az = a.z & (gz - 1);
-            const int bx = b.x & (gx - 1), by = b.y & (gy - 1), bz = b.z & (gz - 1);
-            if (ay == by && az == bz && ax >= 1 && bx >= ax && bx + 1 < gx && bx - ax + 3 <= LK_W) {
-                const uint32_t width = (uint32_t)(bx - ax) + 3u;
-                if (grid_hash(P, bx + 1, ay, az) - grid_hash(P, ax - 1, ay, az) == width - 1u) {
-                    int r = 0;
-                    for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
-                        for (int dy = -1; dy <= 1; ++dy, ++r) lp.key[r] = grid_hash(P, ax - 1, ay + dy, az + dz);
-                    lp.width = width;
-                    lp.cx0 = ax;
-                    ok = 1;
-                }
-            }
+        lp.split = 0x7fffffff;
+        lp.bad = 0;
+        if (eligible) {
+            const Cell a = wrapped(b0);
+            lp.ry[0] = a.y;
+            lp.rz[0] = a.z;
         }
-        lp.ok = ok;
     }
     __syncthreads();
-    if (!lp.ok) return false;
-    const uint32_t width = lp.width;
-    for (uint32_t t = threadIdx.x; t < (uint32_t)R * (LK_W + 1); t += blockDim.x) {
-        const uint32_t r = t / (LK_W + 1), c = t % (LK_W + 1);
-        if (c <= width) lt.first[r][c] = cells.first[lp.key[r] + c];
-        if (c < width) lt.pre[r][c] = cells.pre[lp.key[r] + c];
+    const uint32_t i = b0 + threadIdx.x;
+    Cell mine{0, 0, 0};
+    if (eligible && i <= b1) {
+        mine = wrapped(i);
+        if (mine.y != lp.ry[0] || mine.z != lp.rz[0]) atomicMin(&lp.split, (int)threadIdx.x);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && eligible && lp.split != 0x7fffffff) {
+        const Cell c = wrapped(b0 + (uint32_t)lp.split);
+        lp.ry[1] = c.y;
+        lp.rz[1] = c.z;
+    }
+    __syncthreads();
+    if (eligible && i <= b1 && (int)threadIdx.x > lp.split && (mine.y != lp.ry[1] || mine.z != lp.rz[1]))
+        lp.bad = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int parts = 0;
+        if (eligible && !lp.bad) {
+            const int np = lp.split == 0x7fffffff ? 1 : 2;
+            const int wmax = np == 1 ? LK_W : LK_W / 2;
+            // each part's targets run from cell (ax) to cell (bx) of its row
+            const uint32_t first_t[2] = {b0, np == 2 ? b0 + (uint32_t)lp.split : 0u};
+            const uint32_t last_t[2] = {np == 2 ? b0 + (uint32_t)lp.split - 1u : b1, b1};
+            bool ok = true;
+            for (int pt = 0; pt < np && ok; ++pt) {
+                const Cell a = wrapped(first_t[pt]), b = wrapped(last_t[pt]);
+                const int ax = a.x, bx = b.x, ay = a.y, az = a.z;
+                ok = ax >= 1 && bx >= ax && bx + 1 < gx && bx - ax + 3 <= wmax;
+                if (!ok) break;
+                const uint32_t width = (uint32_t)(bx - ax) + 3u;
+                ok = grid_hash(P, bx + 1, ay, az) - grid_hash(P, ax - 1, ay, az) == width - 1u;
+                if (!ok) break;
+                int r = 0;
+                for (int dz = (DIM == 3 ? -1 : 0); dz <= (DIM == 3 ? 1 : 0); ++dz)
+                    for (int dy = -1; dy <= 1; ++dy, ++r) lp.key[pt][r] = grid_hash(P, ax - 1, ay + dy, az + dz);
+                lp.width[pt] = width;
+                lp.cx0[pt] = ax;
+            }
+            if (ok) parts = np;
+        }
+        lp.parts = parts;
+    }
+    __syncthreads();
+    if (!lp.parts) return false;
+    const uint32_t W = lp.parts == 2 ? (uint32_t)(LK_W / 2) : (uint32_t)LK_W;
+    const uint32_t rows = (uint32_t)(lp.parts * R);
+    for (uint32_t t = threadIdx.x; t < rows * (W + 1u); t += blockDim.x) {
+        const uint32_t rr = t / (W + 1u), c = t % (W + 1u);
+        const uint32_t pt = rr / (uint32_t)R, r = rr % (uint32_t)R;
+        const uint32_t width = lp.width[pt], key = lp.key[pt][r];
+        if (c <= width) lt.first[rr * (W + 1u) + c] = cells.first[key + c];
+        if (c < width) lt.pre[rr * W + c] = cells.pre[key + c];
     }
     __syncthreads();
     return true;
@@ -426,19 +510,23 @@ struct RowLds {
 template <int DIM>
 __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTables& lt,
                                            const float4* __restrict__ pos, RowLds& lds, RowStage& rs) {
+    // blocks staged over one row of cells only (its rows are at stride LK_W + 1)
+    constexpr uint32_t FS = LK_W + 1;
     if (threadIdx.x == 0) {
         int32_t tot = 0;
         for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
-            const uint32_t s = lt.first[ROW_STAGE_FIRST + k][0], e = lt.first[ROW_STAGE_FIRST + k][lp.width];
+            const uint32_t r = (uint32_t)(ROW_STAGE_FIRST + k);
+            const uint32_t s = lt.first[r * FS], e = lt.first[r * FS + lp.width[0]];
             rs.off[k] = tot - (int32_t)s;
             tot += (int32_t)(e - s);
         }
-        rs.ok = DIM == 3 && tot <= ROW_STAGE_CAP;
+        rs.ok = DIM == 3 && lp.parts == 1 && tot <= ROW_STAGE_CAP;
     }
     __syncthreads();
     if (!rs.ok) return;
     for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
-        const uint32_t s = lt.first[ROW_STAGE_FIRST + k][0], e = lt.first[ROW_STAGE_FIRST + k][lp.width];
+        const uint32_t r = (uint32_t)(ROW_STAGE_FIRST + k);
+        const uint32_t s = lt.first[r * FS], e = lt.first[r * FS + lp.width[0]];
         const int32_t d0 = (int32_t)s + rs.off[k];
         for (uint32_t t = threadIdx.x; t < e - s; t += blockDim.x) {
             const float4 v = pos[s + t];
@@ -453,12 +541,13 @@ __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTab
 // lookups, as walk_stencil_pf hands them; with ROWS, the stencil rows that
 // stage_rows copied are read from LDS
 template <int DIM, bool ROWS = false, typename C>
-__device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, int wx, const LookupPlan& lp,
-                                            const LookupTables& lt, const float4* __restrict__ pos, C&& cand,
-                                            const RowStage* rs = nullptr, const RowLds* lds = nullptr) {
+__device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, const Cell& gp,
+                                            const LookupPlan& lp, const LookupTables& lt,
+                                            const float4* __restrict__ pos, C&& cand, const RowStage* rs = nullptr,
+                                            const RowLds* lds = nullptr) {
     constexpr int R = DIM == 3 ? 9 : 3;
     const int q = x_quarter(P, pi.x);
-    const uint32_t dx = (uint32_t)(wx - lp.cx0);
+    const LookupView v = lookup_view<DIM>(P, lp, lt, gp);
     auto run_lds = [&](uint32_t s, uint32_t e, int32_t off) {
         for (uint32_t j = s; j < e; j += 2u) {
             const int32_t l = (int32_t)j + off;  // in [0, staged rows]
@@ -470,11 +559,11 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
         }
     };
     for (int r = 0; r < R; ++r) {
-        uint32_t s = lt.first[r][dx];
-        if (q > 0) s += SubCells::through(lt.pre[r][dx], q - 1);
-        const uint32_t f1 = lt.first[r][dx + 2];
-        const uint4 c1 = lt.pre[r][dx + 2];
-        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt.first[r][dx + 3];
+        const uint4 c0 = v.Pr(r, 0);
+        const uint32_t s = v.F(r, 0) + SubCells::below(c0, q);
+        const uint32_t f1 = v.F(r, 2);
+        const uint4 c1 = v.Pr(r, 2);
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
         if (ROWS && rs->ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS) {
             run_lds(s, e, rs->off[r - ROW_STAGE_FIRST]);
             run_lds(ws, we, rs->off[r - ROW_STAGE_FIRST]);
@@ -487,21 +576,23 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
 
 // ---- in-range masks (staged blocks of the sub-cell walk) ---------------------
 // The density pass tests every candidate of a staged block anyway; it leaves,
-// per target and stencil row r, a 64-bit mask whose bit t says that candidate
-// s + t of the row's main run [s, e) lies within 2h (r^2 < 4h^2, the force
-// pass's own test on the same float operations).  The force pass then pushes a
-// wall-free row's in-range neighbours straight from the mask, in the walk's
-// order, without loading or testing its candidates.  Masks are stored
-// row-major, [r][target] (uint2: low word bits 0..31); a row whose run is
-// longer than MASK_BITS or holds walls is walked as before.
+// per target and stencil row r, a 64-bit mask of which candidates of the
+// row's main run [s, e) lie within 2h (r^2 < 4h^2, the force pass's own test
+// on the same float operations).  The force pass then pushes a wall-free
+// row's in-range neighbours straight from the mask, in the walk's order,
+// without loading or testing its candidates.  Masks are stored row-major,
+// [r][target] (uint2: low word first); a row whose run is longer than
+// MASK_BITS or holds walls is walked as before.
+// Layout: the run is scanned in pairs (t, t + 1) and each pair shifts two bits
+// in from the right, t's bit above t + 1's, so after the K = ceil(L / 2) pairs
+// of a run of L candidates, candidate t sits at bit 2K - 1 - t (a pair past the
+// run's end adds a zero bit).  Bit-reversed, t sits at 64 - 2K + t.
 constexpr int MASK_BITS = 64;
 constexpr int MASK_ROWS = 9;
 
-// the in-range bits of a candidate pair (t, t + 1) of a run, placed at t
-// (t is even); bits past MASK_BITS wrap and are never read
-__device__ __forceinline__ uint64_t mask_pair(uint32_t t, float r2a, float r2b, float r2max) {
-    const uint32_t b = (r2a < r2max ? 1u : 0u) | (r2b < r2max ? 2u : 0u);
-    return (uint64_t)b << (t & 63u);
+__device__ __forceinline__ uint64_t mask_shift_in(uint64_t m, float r2a, float r2b, float r2max) {
+    const uint32_t b = (r2a < r2max ? 2u : 0u) | (r2b < r2max ? 1u : 0u);
+    return (m << 2) + (uint64_t)b;
 }
 
 // density of the target pi over a staged block's lookups, as
@@ -515,7 +606,7 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
     constexpr int R = DIM == 3 ? 9 : 3;
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const int q = x_quarter(P, pi.x);
-    const uint32_t dx = (uint32_t)((gp.x & ((int)P.grid_size[0] - 1)) - lp.cx0);
+    const LookupView v = lookup_view<DIM>(P, lp, lt, gp);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
     float sum = 0.0f;
@@ -529,11 +620,11 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
         term(valid ? d2(pj) : 1e30f);
     };
     for (int r = 0; r < R; ++r) {
-        uint32_t s = lt.first[r][dx];
-        if (q > 0) s += SubCells::through(lt.pre[r][dx], q - 1);
-        const uint32_t f1 = lt.first[r][dx + 2];
-        const uint4 c1 = lt.pre[r][dx + 2];
-        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt.first[r][dx + 3];
+        const uint4 c0 = v.Pr(r, 0);
+        const uint32_t s = v.F(r, 0) + SubCells::below(c0, q);
+        const uint32_t f1 = v.F(r, 2);
+        const uint4 c1 = v.Pr(r, 2);
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
         uint64_t m = 0;
         const bool in_lds = DIM == 3 && rs.ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS;
         if (in_lds) {
@@ -545,7 +636,7 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                 const float rb = j + 1u < e ? d2(make_float4(a1.x, a1.y, lds->z[l + 1], 0.0f)) : 1e30f;
                 term(ra);
                 term(rb);
-                m |= mask_pair(j - s, ra, rb, r2max);
+                m = mask_shift_in(m, ra, rb, r2max);
             }
         } else {
             // as walk_run_sub: two rows per iteration, each load one candidate ahead
@@ -562,7 +653,7 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                     p0 = ld(off + 32u);
                     const float rb = off < end1 ? d2(p1) : 1e30f;
                     term(rb);
-                    m |= mask_pair((off >> 4) - s, ra, rb, r2max);
+                    m = mask_shift_in(m, ra, rb, r2max);
                     off += 32u;
                 } while (off < end);
             }
@@ -609,6 +700,7 @@ struct NbEntry {
 // in the pair loop instead of waiting for the fullest one; a final drain
 // empties the queues.
 constexpr int NB_DRAIN = CMAG_NB_DRAIN;
+static_assert(NB_DRAIN >= 2, "the mask walk pushes two entries between drains");
 
 template <int CAP>
 struct NbQueue {
@@ -622,6 +714,8 @@ struct NbQueue {
         ++count;
     }
     __device__ __forceinline__ bool full() const { return count == CAP; }
+    // fewer than two free entries
+    __device__ __forceinline__ bool nearly_full() const { return count >= CAP - 1; }
     // f(e0, e1, second): the oldest entries two at a time; a lane with an odd
     // number left repeats its last entry with second = false (contributing
     // nothing), so both entries' rows can be loaded before either is used
@@ -670,7 +764,7 @@ __device__ __forceinline__ float density_sum_lookup(const cmag_params& P, const 
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     float sum = 0.0f;
-    walk_lookup<DIM, true>(P, pi, gp.x & ((int)P.grid_size[0] - 1), lp, lt, pos,
+    walk_lookup<DIM, true>(P, pi, gp, lp, lt, pos,
                            [&](uint32_t, const float4& pj, bool valid, float px, float py, float pz, int) {
         const float r2 = valid ? norm2(px - pj.x, py - pj.y, pz - pj.z) : 1e30f;
         const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
@@ -899,32 +993,44 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         if (lp && masks) {
             constexpr int R = DIM == 3 ? 9 : 3;
             const int q = x_quarter(P, pi.x);
-            const uint32_t dx = (uint32_t)((gp.x & ((int)P.grid_size[0] - 1)) - lp->cx0);
+            const LookupView v = lookup_view<DIM>(P, *lp, *lt, gp);
+            // each row's mask is loaded one row ahead of its use
+            uint2 mw = masks[i];
             for (int r = 0; r < R; ++r) {
-                uint32_t s = lt->first[r][dx];
-                const uint4 c0 = lt->pre[r][dx];
-                if (q > 0) s += SubCells::through(c0, q - 1);
-                const uint32_t f1 = lt->first[r][dx + 2];
-                const uint4 c1 = lt->pre[r][dx + 2];
-                const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = lt->first[r][dx + 3];
+                const uint2 cur = mw;
+                if (r + 1 < R) mw = masks[(size_t)(r + 1) * stride + i];
+                const uint4 c0 = v.Pr(r, 0);
+                const uint32_t f0 = v.F(r, 0);
+                const uint32_t s = f0 + SubCells::below(c0, q);
+                const uint32_t f1 = v.F(r, 2);
+                const uint4 c1 = v.Pr(r, 2);
+                const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
                 // walls of the run's first two cells (the last cell's are [ws, we))
-                const uint32_t walls = (f1 - lt->first[r][dx]) - c0.w - lt->pre[r][dx + 1].w;
+                const uint32_t walls = (f1 - f0) - c0.w - v.Pr(r, 1).w;
                 if (walls != 0u || e - s > (uint32_t)MASK_BITS) {
                     walk_run_sub(sortedPos, s, e, pi.x, pi.y, pi.z, 13, cand);
                 } else {
-                    const uint2 mw = masks[(size_t)r * stride + i];
-                    uint64_t m = ((uint64_t)mw.y << 32) | mw.x;
+                    // bit-reversed, candidate t of the run sits at bit 64 - 2K + t,
+                    // i.e. candidate j at bit j - s0; the target's own bit is cleared
+                    uint64_t m = __builtin_bitreverse64(((uint64_t)cur.y << 32) | cur.x);
+                    const uint32_t s0 = s - (64u - ((e - s + 1u) & ~1u));
+                    const uint32_t d = i - s0;
+                    m &= ~(d < 64u ? (1ull << d) : 0ull);
+                    // two neighbours per iteration; the queue keeps room for both
                     while (m != 0u) {
-                        const uint32_t j = s + (uint32_t)__builtin_ctzll(m);
+                        if (__ballot(nq.nearly_full())) nq.drain(NB_DRAIN, pairs);
+                        nq.push(NbEntry::pack(s0 + (uint32_t)__builtin_ctzll(m), 13));
                         m &= m - 1u;
-                        if (j != i) nq.push(NbEntry::pack(j, 13));
-                        if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
+                        if (m != 0u) {
+                            nq.push(NbEntry::pack(s0 + (uint32_t)__builtin_ctzll(m), 13));
+                            m &= m - 1u;
+                        }
                     }
                 }
                 walk_run_sub(sortedPos, ws, we, pi.x, pi.y, pi.z, 13, cand);
             }
         } else if (lp) {
-            walk_lookup<DIM>(P, pi, gp.x & ((int)P.grid_size[0] - 1), *lp, *lt, sortedPos, cand);
+            walk_lookup<DIM>(P, pi, gp, *lp, *lt, sortedPos, cand);
         } else {
             walk_stencil_pf<DIM, HASH, false>(P, pi, gp, cells, sortedPos, cand);
         }
