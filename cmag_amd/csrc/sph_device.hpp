@@ -698,7 +698,8 @@ struct NbEntry {
 // The walk pushes; when any lane's queue is full every active lane drains its
 // NB_DRAIN oldest entries (fewer if it has fewer), which keeps all lanes busy
 // in the pair loop instead of waiting for the fullest one; a final drain
-// empties the queues.
+// empties the queues.  push() has no room check: every site that pushes
+// leaves each queue below CAP before the next site may push.
 constexpr int NB_DRAIN = CMAG_NB_DRAIN;
 static_assert(NB_DRAIN >= 2, "the mask walk pushes two entries between drains");
 
@@ -1026,6 +1027,9 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                             m &= m - 1u;
                         }
                     }
+                    // the last pushes may have filled a queue, and cand pushes
+                    // before it checks: leave every queue below CAP, as cand does
+                    if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
                 }
                 walk_run_sub(sortedPos, ws, we, pi.x, pi.y, pi.z, 13, cand);
             }

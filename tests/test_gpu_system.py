@@ -257,7 +257,9 @@ def test_inrange_masks_change_nothing(device, case):
     on quarter boundaries, ragged counts and the gated 1M dam break."""
     if case == "gated_1m":
         sc = S.dambreak3d(dx=5e-3, hash_order=L.HASH_MORTON, morton_xbits=7)
-        p, pos, vel, steps = sc.params, sc.pos, sc.vel, 3
+        # long enough for rows whose mask walk fills a queue to be followed
+        # by walked rows (the surge meets the floor and the far wall)
+        p, pos, vel, steps = sc.params, sc.pos, sc.vel, 80
     else:
         order, xbits = (L.HASH_LINEAR, 0) if case == "dam3d_linear" else (L.HASH_MORTON, 4)
         sc = S.dambreak3d(dx=0.02, hash_order=order, morton_xbits=xbits)
