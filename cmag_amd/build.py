@@ -84,7 +84,7 @@ def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
-EXAMPLES = ["poiseuille_report", "dambreak_fronts"]
+EXAMPLES = ["poiseuille_report", "dambreak_fronts", "dambreak_dump"]
 
 
 def build_examples() -> list[Path]:

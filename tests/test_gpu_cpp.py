@@ -60,3 +60,58 @@ def test_cpp_dambreak_fronts_match_martin_moyce(device, tmp_path):
     early = t <= 2.0
     assert np.all(np.abs(h[early] - h_exp[early]) < 0.03)
     assert np.all(h[~early] > h_exp[~early]) and np.all(h[~early] - h_exp[~early] < 0.1)
+
+
+DUMP = Path(__file__).resolve().parent.parent / "cmag_amd" / "examples" / "dambreak_dump"
+
+
+def test_cpp_dambreak_dump_matches_oracle(device, oracle, tmp_path):
+    """The reference's particle snapshot dump (DamBreak.Report/dump.h:16-87)
+    on the C++ interface, on a short schedule: relaxed 3 ms behind the right
+    wall, the wall moved twice, snapshots at 4 and 5 ms.  Each file has the
+    reference's header line and the fluid rows "x y z w vx vy" in sorted
+    order; they equal the oracle's state at the same step (same sort
+    permutation).  The files carry six significant digits, so each value may
+    sit half a unit of its sixth digit from the state it prints, on top of the
+    bars of the in-memory multi-step checks: positions 1e-6 and velocities 1e-3
+    of the field's largest value."""
+    import numpy as np
+
+    from cmag_amd.systems import DamBreakSystem
+    assert DUMP.exists(), "build the examples first (python -m cmag_amd.build)"
+    relax, frames = 0.003, (0.004, 0.005)
+    proc = subprocess.run([str(DUMP), str(tmp_path), str(relax), *map(str, frames)], capture_output=True, text=True,
+                          timeout=120)
+    assert proc.returncode == 0, proc.stderr
+    print(proc.stdout)
+    s = DamBreakSystem((128, 128, 1), 1, (512, 256, 4), 1.0 / 256, False)
+    s.reset()
+    p = s.params
+    sim = oracle.Simulation(p, s.getArray(DamBreakSystem.POSITION))
+    while sim.elapsed < np.float32(relax):
+        sim.update(1)
+    for _ in range(2):
+        sim.P.right_boundary = np.float32(np.float32(sim.P.right_boundary) +
+                                          np.float32(128 * 2) * np.float32(p.particle_radius))
+        sim.pos[:] = oracle.shift_right_boundary(sim.P, sim.pos)
+    s.close()
+    for t, name in zip(frames, ("dump0x004.dat", "dump0x005.dat")):
+        while sim.elapsed < np.float32(t):
+            sim.update(1)
+        f = tmp_path / name
+        assert f.read_text().splitlines()[0] == "x y z w density pressure "
+        got = np.loadtxt(f, skiprows=1)
+        rows = sim.pos[sim.index]
+        fl = rows[:, 3] == 4.0
+        want = np.column_stack([rows[fl, :4], sim.vel[sim.index][fl, :2]])
+        assert got.shape == want.shape, (got.shape, want.shape)
+        assert np.all(got[:, 3] == 4.0)
+        def excess(g, w, rel):
+            mag = np.maximum(np.abs(g), np.abs(w))
+            half6 = 0.5 * 10.0 ** (np.floor(np.log10(np.maximum(mag, 1e-30))) - 5)
+            return (np.abs(g - w) - half6).max() / np.abs(w).max(), rel
+
+        for what, (e, rel) in (("positions", excess(got[:, :3], want[:, :3], 1e-6)),
+                               ("velocities", excess(got[:, 4:], want[:, 4:], 1e-3))):
+            print(f"{name}: {got.shape[0]} fluid rows, {what} beyond print rounding {e:.1e} (bar {rel:g})")
+            assert e <= rel, (name, what)
