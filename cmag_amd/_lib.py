@@ -11,6 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libcmag_hip.so"
 
 CMAG_OK = 0
+CMAG_E_INVALID, CMAG_E_HIP, CMAG_E_SIZE, CMAG_E_STATE = -1, -2, -3, -4
 
 
 class CmagError(RuntimeError):
@@ -139,6 +140,8 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_calc_hash": (C.c_int, [_P, _vp, _vp, _vp, _u32, _vp]),
     "cmag_sort_scratch_bytes": (C.c_size_t, [_u32, C.c_int]),
     "cmag_sort_by_key": (C.c_int, [_vp, _vp, _u32, C.c_int, _vp, C.c_size_t, _vp]),
+    "cmag_sort_noalloc_scratch_bytes": (C.c_size_t, []),
+    "cmag_sort_by_key_noalloc": (C.c_int, [_vp, _vp, _u32, C.c_int, _vp]),
     "cmag_reorder_find_cell_start": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "cmag_density": (C.c_int, [_P, _vp, _vp, _vp, _vp, _u32, _vp]),
     "cmag_count_neighbours": (C.c_int, [_P, _vp, _vp, _vp, _vp, _u32, _vp]),

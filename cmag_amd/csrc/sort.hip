@@ -15,6 +15,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <mutex>
+#include <string>
+
 namespace cmag {
 
 constexpr int RS_NT = 256;  // one thread per digit in the per-tile tables
@@ -400,6 +403,65 @@ int radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* alt_keys, uint32_
     return CMAG_OK;
 }
 
+// ---- scratch for callers that cannot allocate ------------------------------
+// cmag_sort_by_key_noalloc keeps cmag_sort_by_key's scratch (alternate keys and
+// values, then the per-tile digit tables) in this file-scope array, sized for
+// CMAG_SORT_NOALLOC_MAX_KEYS keys.  Every GPU the library runs on holds its own
+// copy until the process ends.  On the host the array's name is a stand-in, not
+// a device address: host code only ever uses the address hipGetSymbolAddress
+// returns for the current device, fetched once per device.
+constexpr uint32_t NOALLOC_MAX_KEYS = CMAG_SORT_NOALLOC_MAX_KEYS;
+static_assert(NOALLOC_MAX_KEYS > RS_SMALL_N && NOALLOC_MAX_KEYS % (RS_NT * RS_ITEMS) == 0,
+              "the no-allocation limit sits in the large-tile regime, on a whole tile");
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+constexpr size_t scan_levels_words_c(size_t len) {
+    size_t words = 0;
+    while (len > (size_t)SCAN_TILE) {
+        len = (len + SCAN_TILE - 1) / SCAN_TILE;
+        words += len;
+    }
+    return words;
+}
+
+// cmag_sort_scratch_bytes at a key count in the large-tile regime; the
+// small-tile regime (n <= RS_SMALL_N) needs less than it does at the limit
+constexpr size_t sort_scratch_bytes_large(size_t n) {
+    return round256(n * 2 * sizeof(uint32_t)) +
+           round256((2 * 256 * (n / (RS_NT * RS_ITEMS)) + scan_levels_words_c(256 * (n / (RS_NT * RS_ITEMS)))) *
+                    sizeof(uint32_t));
+}
+constexpr size_t NOALLOC_BYTES = sort_scratch_bytes_large(NOALLOC_MAX_KEYS);
+static_assert(sort_scratch_bytes_large(NOALLOC_MAX_KEYS) >=
+                  round256((size_t)RS_SMALL_N * 8) +
+                      round256((2 * 256 * (RS_SMALL_N / (RS_NT * RS_ITEMS_SMALL)) +
+                                scan_levels_words_c(256 * (RS_SMALL_N / (RS_NT * RS_ITEMS_SMALL)))) *
+                               sizeof(uint32_t)),
+              "the array covers the small-tile regime too");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static void* noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device
+static int noalloc_scratch(void** out) {
+    int dev = -1;
+    CMAG_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CMAG_E_INVALID, "device ordinal beyond 63");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[dev]) {
+        void* p = nullptr;
+        CMAG_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CMAG_E_HIP, "hipGetSymbolAddress returned a null address for the sort scratch");
+        noalloc_ptr[dev] = p;
+    }
+    *out = noalloc_ptr[dev];
+    return CMAG_OK;
+}
+
 }  // namespace cmag
 
 using namespace cmag;
@@ -431,6 +493,22 @@ int cmag_sort_by_key(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, v
         CMAG_TRY(launch_copy_u32(vals, alt_vals, n, s));
     }
     return CMAG_OK;
+}
+
+size_t cmag_sort_noalloc_scratch_bytes(void) { return NOALLOC_BYTES; }
+
+int cmag_sort_by_key_noalloc(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, cmag_stream_t stream) {
+    if (n && (!keys || !vals)) return fail(CMAG_E_INVALID, "cmag_sort_by_key_noalloc: null pointer");
+    if (n > NOALLOC_MAX_KEYS)
+        return fail(CMAG_E_SIZE, "cmag_sort_by_key_noalloc: n = " + std::to_string(n) +
+                                     " keys is above the limit of " + std::to_string(NOALLOC_MAX_KEYS) +
+                                     " (CMAG_SORT_NOALLOC_MAX_KEYS); use cmag_sort_by_key with a scratch buffer");
+    if (n < 2 || key_bits == 0) return CMAG_OK;
+    if (cmag_sort_scratch_bytes(n, key_bits) > NOALLOC_BYTES)
+        return fail(CMAG_E_SIZE, "cmag_sort_by_key_noalloc: scratch array smaller than the sort needs");
+    void* ws = nullptr;
+    CMAG_TRY(noalloc_scratch(&ws));
+    return cmag_sort_by_key(keys, vals, n, key_bits, ws, NOALLOC_BYTES, stream);
 }
 
 }  // extern "C"

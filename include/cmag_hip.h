@@ -104,6 +104,21 @@ size_t cmag_sort_scratch_bytes(uint32_t n, int key_bits);
 int cmag_sort_by_key(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, void* scratch,
                      size_t scratch_bytes, cmag_stream_t stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  cmag_sort_by_key with its scratch in a file-scope device array
+ * of cmag_sort_noalloc_scratch_bytes() bytes (35,651,840: alternate keys and
+ * values plus the digit tables for CMAG_SORT_NOALLOC_MAX_KEYS keys), held by
+ * every GPU the library runs on until the process ends.  Calls no hipMalloc,
+ * hipFree or allocating library routine; n above the limit returns CMAG_E_SIZE.
+ * Sorts on the current device (the stream must belong to it).  All calls on
+ * one GPU share that GPU's array, so calls that overlap on one GPU (other
+ * streams, other host threads) race; calls in one stream order are safe.
+ * Nothing in the array needs to start at zero. */
+#define CMAG_SORT_NOALLOC_MAX_KEYS 4194304u
+size_t cmag_sort_noalloc_scratch_bytes(void);
+int cmag_sort_by_key_noalloc(uint32_t* keys, uint32_t* vals, uint32_t n, int key_bits, cmag_stream_t stream);
+
 /* reorderDataAndFindCellStart: fluidSystem.cu:150-188 (incl. its cudaMemset of
  * cellStart).  cellEnd of empty cells is not written. */
 int cmag_reorder_find_cell_start(uint32_t* cellStart, uint32_t* cellEnd, float* sortedPos,
