@@ -103,6 +103,36 @@ def measured_traffic(workload_key: str, kernel: str) -> float | None:
         return None
 
 
+# VALU issue peak: one wave64 VALU instruction per CU per cycle (four SIMDs of
+# 16 lanes, four cycles per wave64 instruction), 256 CUs at 2.4 GHz
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
+
+
+def valu_roofline(workload_key: str, per_stage: dict) -> list | None:
+    """Issue-rate roofline of the neighbour passes: the VALU instructions per
+    launch from the committed counter summary of this workload over the live
+    launch time, against the issue peak"""
+    f = ROOT / "profiles" / "valu.json"
+    if not f.exists():
+        return None
+    try:
+        rec = json.loads(f.read_text()).get(workload_key, {}).get("kernels", {})
+    except ValueError:
+        return None
+    out = []
+    for stage in ("force", "density"):
+        k = rec.get(stage)
+        if not k or stage not in per_stage or not per_stage[stage]["avg_ms"]:
+            continue
+        achieved = k["valu_insts_per_launch"] / (per_stage[stage]["avg_ms"] * 1e-3)
+        out.append({"bound": "valu", "kernel": stage, "achieved": round(achieved / 1e9, 1),
+                    "peak": round(VALU_PEAK_WAVE_INSTS / 1e9, 1), "unit": "G wave64 VALU instr/s",
+                    "frac": round(achieved / VALU_PEAK_WAVE_INSTS, 3),
+                    "valu_insts_per_wave": k["valu_insts_per_wave"],
+                    "lane_utilization": k.get("valu_lane_utilization")})
+    return out or None
+
+
 def bench_batch(steps: int = 1000, plain_steps: int = 20, tune_cells: bool = True) -> dict:
     """The batched workload: 256 independent 2-D channels (half Poiseuille, half
     peristaltic, 1k..64k particles each) stepped together in one launch sequence
@@ -310,6 +340,11 @@ def main() -> int:
     ap.add_argument("--relax", type=float, default=0.05,
                     help="seconds of damped settling behind the gate before the warm-up steps (0: start from the "
                          "lattice)")
+    ap.add_argument("--alpha", type=float, default=0.01,
+                    help="artificial viscosity of the timed run (the workload names 0.01; the reference hard-codes "
+                         "0.38, fluid_kernel.cu:229)")
+    ap.add_argument("--relax-eager", action="store_true",
+                    help="step the settling phase eagerly rather than by graph replay (for counter profiles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-channel comparison")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -342,7 +377,7 @@ def main() -> int:
     order = L.HASH_MORTON if args.hash == "morton" else L.HASH_LINEAR
     hash_name = (f"Z-order over {1 << args.morton_xbits}x1x1 cell bricks" if args.hash == "morton"
                  else "linear")
-    sc = S.dambreak3d(dx=args.dx, hash_order=order, morton_xbits=args.morton_xbits)
+    sc = S.dambreak3d(dx=args.dx, hash_order=order, morton_xbits=args.morton_xbits, alpha=args.alpha)
     n_total = sc.pos.shape[0]
     print(f"# workload: {sc.num_fluid} fluid + {sc.meta['num_wall']} wall particles, grid "
           f"{tuple(sc.params.grid_size)}, hash {args.hash}", file=sys.stderr, flush=True)
@@ -352,7 +387,15 @@ def main() -> int:
         # the column settles behind its gate first: a lattice start has a
         # surface transient of several m/s in its first hundred steps
         t_relax = time.perf_counter()
-        S.relax_gated(sysobj, sc, seconds=args.relax, alpha=2.0, zero_every=0.005, free_last=0.005)
+        last = [time.perf_counter()]
+
+        def report(done, total):
+            if time.perf_counter() - last[0] > 20.0:
+                last[0] = time.perf_counter()
+                print(f"# relaxing: {done} / {total} steps", file=sys.stderr, flush=True)
+
+        S.relax_gated(sysobj, sc, seconds=args.relax, alpha=2.0, zero_every=0.005, free_last=0.005, report=report,
+                      use_graph=not args.relax_eager)
         sysobj.synchronize()
         print(f"# relaxed {args.relax} s behind the gate in {time.perf_counter() - t_relax:.1f} s",
               file=sys.stderr, flush=True)
@@ -379,8 +422,10 @@ def main() -> int:
         copy_probe()
     wall_s = t1 - t0
     ms_per_step = 1000.0 * wall_s / args.steps
-    vmax_after = round(float(np.linalg.norm(sysobj.getArray(L.ARRAY_VELOCITY, 0, sc.num_fluid)[:, :3],
-                                            axis=1).max()), 3)
+    speed = np.linalg.norm(sysobj.getArray(L.ARRAY_VELOCITY, 0, sc.num_fluid)[:, :3], axis=1)
+    state_after = {"t_s": round(sysobj.getElapsedTime(), 5), "vmax": round(float(speed.max()), 3),
+                   "v99": round(float(np.quantile(speed, 0.99)), 3), "v50": round(float(np.median(speed)), 4)}
+    del speed
 
     per_stage = {}
     for name, (ms, cnt) in stages.items():
@@ -412,10 +457,11 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": (f"3-D dam break, {sc.num_fluid:,} fluid + {sc.meta['num_wall']:,} wall particles, "
-                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {hash_name} cell hash, Tait, alpha=0.01; "
+                         f"dx={args.dx:.6g}, h=1.3dx, cell=2h, {hash_name} cell hash, Tait, alpha={args.alpha:g}; "
                          + (f"the column behind its gate after {args.relax} s of damped settling"
                             if relaxed else "the column from rest behind its gate")),
-            "fluid_vmax_after": vmax_after,
+            # fluid speeds (m/s) after the timed steps
+            "fluid_state_after": state_after,
             "num_fluid": sc.num_fluid,
             "num_total": n_total,
             "grid": list(sc.params.grid_size),
@@ -437,6 +483,9 @@ def main() -> int:
             "traffic": traffic,
         },
     }
+    valu = valu_roofline(workload_key, per_stage)
+    if valu:
+        result["valu_roofline"] = valu
     if not args.no_batch:
         result["batched"] = bench_batch(steps=args.batch_steps)
     if not args.no_adjoint:

@@ -597,7 +597,7 @@ __device__ __forceinline__ uint64_t mask_shift_in(uint64_t m, float r2a, float r
 
 // density of the target pi over a staged block's lookups, as
 // density_sum_lookup, writing the in-range masks of its rows' main runs to
-// masks[r * stride + i]
+// masks[r * stride + i] (fluid targets only)
 template <int DIM>
 __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const float4& pi, const LookupPlan& lp,
                                                    const LookupTables& lt, const RowStage& rs, const RowLds* lds,
@@ -609,6 +609,8 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
     const LookupView v = lookup_view<DIM>(P, lp, lt, gp);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
+    // the force pass reads the masks of fluid targets only
+    const bool fluid = pi.w == CMAG_DB_FLUID;
     float sum = 0.0f;
     auto term = [&](float r2) {
         const float t = fminf(fmaxf(fmaf(half_inv_h, fsqrt(r2), 1.0f), 0.0f), 1.0f);
@@ -658,7 +660,7 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                 } while (off < end);
             }
         }
-        masks[(size_t)r * stride + i] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
+        if (fluid) masks[(size_t)r * stride + i] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
         if (in_lds) {
             const int32_t off = rs.off[r - ROW_STAGE_FIRST];
             for (uint32_t j = ws; j < we; j += 2u) {

@@ -395,12 +395,14 @@ MARTIN_MOYCE_SURGE = ((0.00, 1.00), (0.41, 1.11), (0.84, 1.22), (1.19, 1.44), (1
 
 
 def relax_gated(system, sc: Scenario, seconds: float = 0.3, alpha: float = 2.0, zero_every: float = 0.02,
-                free_last: float = 0.02) -> None:
+                free_last: float = 0.02, report=None, use_graph: bool = True) -> None:
     """Let a gated dam break settle behind its gate: stepping with a strong
     artificial viscosity and the velocities set to zero every zero_every
     seconds (kinetic damping), the last free_last seconds undamped, so that
     what is left is the column's own residual motion.  The parameter block's
-    alpha is restored afterwards; the clock keeps running."""
+    alpha is restored afterwards; the clock keeps running.  report(done,
+    total), if given, is called after every batch of steps; use_graph=False
+    steps eagerly instead of replaying a captured step."""
     P0 = system.params
     P = P0.copy()
     P.alpha = alpha
@@ -413,8 +415,10 @@ def relax_gated(system, sc: Scenario, seconds: float = 0.3, alpha: float = 2.0, 
     k = 0
     while k < total:
         m = min(every, total - k)
-        system.run(m, use_graph=True)
+        system.run(m, use_graph=use_graph)
         k += m
+        if report is not None:
+            report(k, total)
         if k < total - free:
             system.setArray(L.ARRAY_VELOCITY, zeros)
             system.setArray(L.ARRAY_VELOCITYLEAPFROG, zeros)
