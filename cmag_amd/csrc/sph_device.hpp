@@ -351,7 +351,7 @@ __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const floa
 // A block whose targets lie in one row of cells along x, or in two rows (the
 // end of one and the start of the next in key order), with each row's stencil
 // x-range [cx0 - 1, cx1 + 1] inside the grid, on consecutive keys and at most
-// LK_W cells wide (LK_W / 2 for each of two rows), copies the compact offsets
+// LK_W cells wide (both rows together when there are two), copies the compact offsets
 // and quarter counts of those cells in the 9 (2-D: 3) stencil rows of each of
 // its rows into LDS once; its targets then find their runs there instead of
 // with five loads per stencil row.  The runs and their order are the global
@@ -368,8 +368,9 @@ struct LookupPlan {
     int bad;             // a target outside the two rows
 };
 
-// one part: rows r = 0..8 at first[r * (W + 1) + c], pre[r * W + c];
-// two parts: W = LK_W / 2, part 1's rows follow part 0's
+// part p (width w_p) keeps its stencil rows r at first[f_p + r * (w_p + 1) + c]
+// and pre[q_p + r * w_p + c]: f_0 = q_0 = 0, part 1 after part 0 (f_1 =
+// R (w_0 + 1), q_1 = R w_0); w_0 + w_1 <= LK_W
 struct LookupTables {
     uint32_t first[18 * (LK_W / 2 + 1)];
     uint4 pre[9 * LK_W];
@@ -392,13 +393,13 @@ __device__ __forceinline__ LookupView lookup_view(const cmag_params& P, const Lo
     const int wx = gp.x & ((int)P.grid_size[0] - 1), wy = gp.y & ((int)P.grid_size[1] - 1),
               wz = gp.z & ((int)P.grid_size[2] - 1);
     const int part = (lp.parts == 2 && wy == lp.ry[1] && wz == lp.rz[1]) ? 1 : 0;
-    const uint32_t W = lp.parts == 2 ? (uint32_t)(LK_W / 2) : (uint32_t)LK_W;
+    const uint32_t w0 = lp.width[0], w = part ? lp.width[1] : w0;
     const uint32_t dx = (uint32_t)(wx - lp.cx0[part]);
     LookupView v;
-    v.fs = W + 1u;
-    v.ps = W;
-    v.first = lt.first + (uint32_t)(part * R) * v.fs + dx;
-    v.pre = lt.pre + (uint32_t)(part * R) * v.ps + dx;
+    v.fs = w + 1u;
+    v.ps = w;
+    v.first = lt.first + (part ? (uint32_t)R * (w0 + 1u) : 0u) + dx;
+    v.pre = lt.pre + (part ? (uint32_t)R * w0 : 0u) + dx;
     return v;
 }
 
@@ -449,7 +450,7 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
         int parts = 0;
         if (eligible && !lp.bad) {
             const int np = lp.split == 0x7fffffff ? 1 : 2;
-            const int wmax = np == 1 ? LK_W : LK_W / 2;
+            uint32_t wsum = 0;
             // each part's targets run from cell (ax) to cell (bx) of its row
             const uint32_t first_t[2] = {b0, np == 2 ? b0 + (uint32_t)lp.split : 0u};
             const uint32_t last_t[2] = {np == 2 ? b0 + (uint32_t)lp.split - 1u : b1, b1};
@@ -457,9 +458,12 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
             for (int pt = 0; pt < np && ok; ++pt) {
                 const Cell a = wrapped(first_t[pt]), b = wrapped(last_t[pt]);
                 const int ax = a.x, bx = b.x, ay = a.y, az = a.z;
-                ok = ax >= 1 && bx >= ax && bx + 1 < gx && bx - ax + 3 <= wmax;
+                ok = ax >= 1 && bx >= ax && bx + 1 < gx && bx - ax + 3 <= LK_W;
                 if (!ok) break;
                 const uint32_t width = (uint32_t)(bx - ax) + 3u;
+                wsum += width;
+                ok = wsum <= (uint32_t)LK_W;
+                if (!ok) break;
                 ok = grid_hash(P, bx + 1, ay, az) - grid_hash(P, ax - 1, ay, az) == width - 1u;
                 if (!ok) break;
                 int r = 0;
@@ -474,14 +478,15 @@ __device__ __forceinline__ bool stage_lookups(const cmag_params& P, const float4
     }
     __syncthreads();
     if (!lp.parts) return false;
-    const uint32_t W = lp.parts == 2 ? (uint32_t)(LK_W / 2) : (uint32_t)LK_W;
-    const uint32_t rows = (uint32_t)(lp.parts * R);
-    for (uint32_t t = threadIdx.x; t < rows * (W + 1u); t += blockDim.x) {
-        const uint32_t rr = t / (W + 1u), c = t % (W + 1u);
-        const uint32_t pt = rr / (uint32_t)R, r = rr % (uint32_t)R;
-        const uint32_t width = lp.width[pt], key = lp.key[pt][r];
-        if (c <= width) lt.first[rr * (W + 1u) + c] = cells.first[key + c];
-        if (c < width) lt.pre[rr * W + c] = cells.pre[key + c];
+    // part 0's entries first, then part 1's; entry t of first[] is t
+    const uint32_t w0 = lp.width[0], w1 = lp.parts == 2 ? lp.width[1] : 0u;
+    const uint32_t n0 = (uint32_t)R * (w0 + 1u), total = n0 + (lp.parts == 2 ? (uint32_t)R * (w1 + 1u) : 0u);
+    for (uint32_t t = threadIdx.x; t < total; t += blockDim.x) {
+        const uint32_t pt = t >= n0 ? 1u : 0u, loc = pt ? t - n0 : t, w = pt ? w1 : w0;
+        const uint32_t r = loc / (w + 1u), c = loc - r * (w + 1u);
+        const uint32_t key = lp.key[pt][r];
+        lt.first[t] = cells.first[key + c];
+        if (c < w) lt.pre[(pt ? (uint32_t)R * w0 : 0u) + r * w + c] = cells.pre[key + c];
     }
     __syncthreads();
     return true;
@@ -510,8 +515,8 @@ struct RowLds {
 template <int DIM>
 __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTables& lt,
                                            const float4* __restrict__ pos, RowLds& lds, RowStage& rs) {
-    // blocks staged over one row of cells only (its rows are at stride LK_W + 1)
-    constexpr uint32_t FS = LK_W + 1;
+    // blocks staged over one row of cells only (its rows are at stride width + 1)
+    const uint32_t FS = lp.width[0] + 1u;
     if (threadIdx.x == 0) {
         int32_t tot = 0;
         for (int k = 0; k < ROW_STAGE_ROWS; ++k) {
