@@ -581,19 +581,29 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
 
 // ---- in-range masks (staged blocks of the sub-cell walk) ---------------------
 // The density pass tests every candidate of a staged block anyway; it leaves,
-// per target and stencil row r, a 64-bit mask of which candidates of the
+// per target and stencil row r, a 64-bit mask of which fluid candidates of the
 // row's main run [s, e) lie within 2h (r^2 < 4h^2, the force pass's own test
-// on the same float operations).  The force pass then pushes a wall-free
-// row's in-range neighbours straight from the mask, in the walk's order,
-// without loading or testing its candidates.  Masks are stored row-major,
-// [r][target] (uint2: low word first); a row whose run is longer than
-// MASK_BITS or holds walls is walked as before.
+// on the same float operations; the walls in the run, which are those of its
+// first two cells, are cleared).  The force pass then pushes the row's in-range
+// fluid neighbours straight from the mask, in the walk's order, without loading
+// or testing those candidates, and walks the walls apart.  Masks are stored
+// row-major, [r][target] (uint2: low word first); a row whose run is longer
+// than MASK_BITS is walked as before.
 // Layout: the run is scanned in pairs (t, t + 1) and each pair shifts two bits
 // in from the right, t's bit above t + 1's, so after the K = ceil(L / 2) pairs
 // of a run of L candidates, candidate t sits at bit 2K - 1 - t (a pair past the
 // run's end adds a zero bit).  Bit-reversed, t sits at 64 - 2K + t.
 constexpr int MASK_BITS = 64;
 constexpr int MASK_ROWS = 9;
+
+// the mask bits of run candidates t in [lo, hi) for a run of K2 = 2K <= 64
+// shifted-in candidates (t at bit K2 - 1 - t)
+__device__ __forceinline__ uint64_t mask_bits(uint32_t K2, uint32_t lo, uint32_t hi) {
+    if (hi <= lo) return 0ull;
+    const uint32_t len = hi - lo;
+    const uint64_t ones = len >= 64u ? ~0ull : ((1ull << len) - 1ull);
+    return ones << (K2 - hi);
+}
 
 __device__ __forceinline__ uint64_t mask_shift_in(uint64_t m, float r2a, float r2b, float r2max) {
     const uint32_t b = (r2a < r2max ? 2u : 0u) | (r2b < r2max ? 1u : 0u);
@@ -628,7 +638,8 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
     };
     for (int r = 0; r < R; ++r) {
         const uint4 c0 = v.Pr(r, 0);
-        const uint32_t s = v.F(r, 0) + SubCells::below(c0, q);
+        const uint32_t f0 = v.F(r, 0);
+        const uint32_t s = f0 + SubCells::below(c0, q);
         const uint32_t f1 = v.F(r, 2);
         const uint4 c1 = v.Pr(r, 2);
         const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
@@ -665,6 +676,12 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                 } while (off < end);
             }
         }
+        // the walls of the run's first two cells are walked by the force pass
+        // (every wall of the stencil acts): their bits are cleared
+        const uint32_t fx = v.F(r, 1), cxw = v.Pr(r, 1).w;
+        const uint32_t K2 = (e - s + 1u) & ~1u;
+        if ((f1 - f0) - c0.w - cxw != 0u && K2 <= (uint32_t)MASK_BITS)
+            m &= ~(mask_bits(K2, f0 + c0.w - s, fx - s) | mask_bits(K2, fx + cxw - s, f1 - s));
         if (fluid) masks[(size_t)r * stride + i] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
         if (in_lds) {
             const int32_t off = rs.off[r - ROW_STAGE_FIRST];
@@ -930,10 +947,11 @@ struct ForceSum {
 // density pass's results: a fluid row holds -rho in pos.w, sign bit set (a wall
 // row keeps its type flag, 0..3, sign bit clear) and p / rho^2 in vel.w, so a
 // pair term gathers two rows instead of three and measures is not read.
-// With masks (staged blocks of the sub-cell walk), the in-range neighbours of a
-// row whose main run holds no walls and at most MASK_BITS candidates come from
-// the density pass's mask instead of a walk; they are pushed in the walk's
-// order, so the sums are the same bit for bit.
+// With masks (staged blocks of the sub-cell walk), the in-range fluid
+// neighbours of a row whose main run holds at most MASK_BITS candidates come
+// from the density pass's mask instead of a walk, and the walls of its first two
+// cells are walked on their own; both keep the walk's order (pair terms and
+// wall terms accumulate apart), so the sums are the same bit for bit.
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -997,6 +1015,13 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
         }
         if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
     };
+    // a run of wall rows only: every one acts (Lennard-Jones), none is queued
+    auto wall_only = [&](uint32_t, const float4& pj, bool valid, float px, float py, float pz, int) {
+        if (valid) {
+            const float rx = px - pj.x, ry = py - pj.y, rz = pz - pj.z;
+            F.wall(rx, ry, rz, norm2(rx, ry, rz));
+        }
+    };
     if constexpr (Cells::SUB) {
         if (lp && masks) {
             constexpr int R = DIM == 3 ? 9 : 3;
@@ -1014,10 +1039,18 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                 const uint4 c1 = v.Pr(r, 2);
                 const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
                 // walls of the run's first two cells (the last cell's are [ws, we))
-                const uint32_t walls = (f1 - f0) - c0.w - v.Pr(r, 1).w;
-                if (walls != 0u || e - s > (uint32_t)MASK_BITS) {
+                const uint32_t cxw = v.Pr(r, 1).w;
+                const uint32_t walls = (f1 - f0) - c0.w - cxw;
+                if (e - s > (uint32_t)MASK_BITS) {
                     walk_run_sub(sortedPos, s, e, pi.x, pi.y, pi.z, 13, cand);
                 } else {
+                    if (walls != 0u) {
+                        // the walls of cells x - 1 and x in run order; the
+                        // mask holds the fluid only
+                        const uint32_t fx = v.F(r, 1);
+                        walk_run_sub(sortedPos, f0 + c0.w, fx, pi.x, pi.y, pi.z, 13, wall_only);
+                        walk_run_sub(sortedPos, fx + cxw, f1, pi.x, pi.y, pi.z, 13, wall_only);
+                    }
                     // bit-reversed, candidate t of the run sits at bit 64 - 2K + t,
                     // i.e. candidate j at bit j - s0; the target's own bit is cleared
                     uint64_t m = __builtin_bitreverse64(((uint64_t)cur.y << 32) | cur.x);
