@@ -141,6 +141,15 @@ SIGNATURES: dict[str, tuple] = {
     "cmag_sort_scratch_bytes": (C.c_size_t, [_u32, C.c_int]),
     "cmag_sort_by_key": (C.c_int, [_vp, _vp, _u32, C.c_int, _vp, C.c_size_t, _vp]),
     "cmag_sort_noalloc_scratch_bytes": (C.c_size_t, []),
+    "cmag_system_color_buffer": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "cmag_system_coloring": (C.c_int, [_vp]),
+    "cmag_fill_colors": (C.c_int, [_P, _vp, _vp, _u32, _vp]),
+    "cmag_render_view_default": (None, [_vp]),
+    "cmag_renderer_create": (C.c_int, [_u32, _u32, C.POINTER(_vp)]),
+    "cmag_renderer_destroy": (C.c_int, [_vp]),
+    "cmag_renderer_display": (C.c_int, [_vp, _vp, _vp, _vp, _u32, _vp]),
+    "cmag_renderer_image": (_vp, [_vp]),
+    "cmag_renderer_read": (C.c_int, [_vp, _vp, _vp]),
     "cmag_sort_by_key_noalloc": (C.c_int, [_vp, _vp, _u32, C.c_int, _vp]),
     "cmag_reorder_find_cell_start": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp]),
     "cmag_density": (C.c_int, [_P, _vp, _vp, _vp, _vp, _u32, _vp]),

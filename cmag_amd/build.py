@@ -34,6 +34,7 @@ SOURCES = [
     "integrate.hip",
     "batch.hip",
     "adjoint.hip",
+    "render.hip",
 ]
 
 HIPCC_FLAGS = [
@@ -84,7 +85,7 @@ def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
-EXAMPLES = ["poiseuille_report", "dambreak_fronts", "dambreak_dump"]
+EXAMPLES = ["poiseuille_report", "dambreak_fronts", "dambreak_dump", "dambreak_render"]
 
 
 def build_examples() -> list[Path]:

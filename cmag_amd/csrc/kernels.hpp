@@ -14,6 +14,8 @@ namespace cmag {
 struct StepState;
 
 int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// the demos' colour buffer of a system's kind (render.hip)
+int launch_fill_colors(const cmag_params& P, const float* pos4, float* color4, uint32_t n, hipStream_t s);
 int launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, uint32_t* index,
                      uint32_t n, hipStream_t s);

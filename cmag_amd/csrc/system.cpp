@@ -49,6 +49,8 @@ struct cmag_system {
     static constexpr size_t MASK_WORDS = 18;
     bool use_masks = true;
     uint32_t* masks = nullptr;
+    // the demos' colour buffer (float4 RGBA per particle), made on first use
+    float* color = nullptr;
 
     float elapsed = 0.0f;
     // Peristalsis boundary-configuration state (PeristalsisSystem.cpp:330-343):
@@ -95,7 +97,7 @@ struct cmag_system {
         void* bufs[] = {pos, vel, velLF, acc, presF, measures, sortedPos, sortedVel, predicted,
                         hash, index, hashAlt, indexAlt, cellStart, cellEnd, cellFirst, scanWs, sortWs,
                         gaccU, gaccS, gposS, gvelS, grho, dState, posW, velW, indexW, w2r, preW, bucketR,
-                        masks};
+                        masks, color};
         for (void* b : bufs)
             if (b) (void)hipFree(b);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -513,6 +515,7 @@ int cmag_system_reset(cmag_system* s) {
         CMAG_TRY(s->set_array(CMAG_ARRAY_VELOCITYLEAPFROG, zero.data(), 0, (int)n));
         CMAG_TRY(s->set_array(CMAG_ARRAY_PREDICTEDPOSITION, zero.data(), 0, (int)n));
         s->P.particle_mass = s->P.rest_density / peristalsis_lattice_sum(s->P, hpos.data(), n);
+        if (s->color) CMAG_TRY(launch_fill_colors(s->P, s->pos, s->color, n, s->stream));
         return CMAG_OK;
     }
     // DamBreakSystem::reset fluidSystem.cpp:361-380; PoiseuilleFlowSystem::reset
@@ -529,6 +532,7 @@ int cmag_system_reset(cmag_system* s) {
         const cmag_params& P = s->P;
         s->P.right_boundary = P.world_origin[0] + (P.boundary_offset + P.fluid_size[0]) * 2 * P.particle_radius;
     }
+    if (s->color) CMAG_TRY(launch_fill_colors(s->P, s->pos, s->color, n, s->stream));
     return CMAG_OK;
 }
 
@@ -696,6 +700,24 @@ int cmag_system_remove_right_boundary(cmag_system* s) {
 }
 
 // PoiseuilleFlowSystem::changeGravity, poiseuilleFlowSystem.cpp:207-210
+int cmag_system_color_buffer(cmag_system* s, void** dev_color4) {
+    if (!s || !dev_color4) return fail(CMAG_E_INVALID, "cmag_system_color_buffer: null pointer");
+    if (!s->color) {
+        CMAG_HIP_TRY(hipMalloc((void**)&s->color, sizeof(float) * 4 * (size_t)(s->n ? s->n : 1)));
+        CMAG_TRY(launch_fill_colors(s->P, s->pos, s->color, s->n, s->stream));
+    }
+    *dev_color4 = s->color;
+    return CMAG_OK;
+}
+
+int cmag_system_coloring(cmag_system* s) {
+    if (!s) return fail(CMAG_E_INVALID, "null system");
+    if (s->P.system != CMAG_SYS_PERISTALSIS) return fail(CMAG_E_INVALID, "Coloring() is PeristalsisSystem's");
+    void* c = nullptr;
+    if (s->color) return launch_fill_colors(s->P, s->pos, s->color, s->n, s->stream);
+    return cmag_system_color_buffer(s, &c);
+}
+
 int cmag_system_change_gravity(cmag_system* s) {
     if (!s) return fail(CMAG_E_INVALID, "null system");
     s->P.gravity[1] *= -1.0f;

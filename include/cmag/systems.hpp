@@ -42,7 +42,13 @@ public:
     unsigned int getColorBuffer() const { return 0u; }
     void* getCudaPosVBO() const { return ptr(CMAG_ARRAY_POSITION); }
     void* getCudaVelVBO() const { return ptr(CMAG_ARRAY_VELOCITY); }
-    void* getCudaColorVBO() const { return nullptr; }
+    // the colour buffer the reference keeps for its GL colour VBO (float4 RGBA
+    // per particle), made on first call (cmag_system_color_buffer)
+    void* getCudaColorVBO() const {
+        void* c = nullptr;
+        check(cmag_system_color_buffer(s_, &c), "getCudaColorVBO");
+        return c;
+    }
     void* getCudaHash() const { return ptr(CMAG_ARRAY_HASH); }
     void* getCudaIndex() const { return ptr(CMAG_ARRAY_INDEX); }
     void* getCudaSortedPosition() const { return ptr(CMAG_ARRAY_SORTEDPOS); }
@@ -193,8 +199,8 @@ public:
     void* getPredictedPos() const { return ptr(CMAG_ARRAY_PREDICTEDPOSITION); }
     void* getLeapFrogVelocity() const { return ptr(CMAG_ARRAY_VELOCITYLEAPFROG); }
     float getradius() const { return p().particle_radius; }
-    // the reference's Coloring() fills a GL color buffer; there is none here
-    void Coloring() {}
+    // PeristalsisSystem::Coloring into the colour buffer of getCudaColorVBO()
+    void Coloring() { cmag_cpp::check(cmag_system_coloring(s_), "Coloring"); }
 
 private:
     static int id(ParticleArray a) {

@@ -267,6 +267,55 @@ int cmag_system_change_right_boundary(cmag_system* s);
 int cmag_system_remove_right_boundary(cmag_system* s);
 int cmag_system_change_gravity(cmag_system* s);
 
+/* ---- visualisation without OpenGL ------------------------------------------
+ * The colour buffer the reference fills for its GL colour VBO (float4 RGBA per
+ * particle, unsorted order), owned by the System, made on first call and
+ * refilled by reset(): DamBreakSystem::_initialize fluidSystem.cpp:160-181
+ * (colorRamp of 0.5 for the fluid, 0.2 for the first-type walls, 0.7 for the
+ * rest; colorRamp :97-114), PoiseuilleFlowSystem::_initialize
+ * poiseuilleFlowSystem.cpp:148-169, and for PeristalsisSystem its Coloring().
+ * The device pointer is borrowed (getCudaColorVBO, fluidSystem.h:42). */
+int cmag_system_color_buffer(cmag_system* s, void** dev_color4);
+/* PeristalsisSystem::Coloring, PeristalsisSystem.cpp:409-442: walls black,
+ * fluid left of worldOrigin.x + 4 radius red (255, 0, 0), the rest white
+ * (255, 255, 255), from the current positions; CMAG_E_INVALID for the other
+ * systems, which have no Coloring(). */
+int cmag_system_coloring(cmag_system* s);
+/* the same rules as a stage launcher, by the parameter block's system kind */
+int cmag_fill_colors(const cmag_params* p, const float* pos4, float* color4, uint32_t n, cmag_stream_t stream);
+
+/* The demos' particle renderer (DamBreak.Demo/render_particles.cpp:84-96 and
+ * the sphere pixel shader of shaders.cpp, with the camera of main.cpp:19,
+ * 109-114, 137), drawn into an RGBA8 image in device memory instead of a GL
+ * window: every particle is a sprite of point_radius * (h / tan(fov / 2)) /
+ * (eye distance) pixels (at most max_point_size), each pixel inside its disc
+ * shaded colour * max(0, L . N), L = (0.577, 0.577, 0.577); the nearest
+ * sprite wins a pixel (equal depths: the lower particle index, as GL_LESS in
+ * draw order); a particle whose centre lies outside the view volume is not
+ * drawn.  The image is stored top row first. */
+typedef struct cmag_render_view {
+    float translate[3];   /* glTranslatef (the demo's camera_trans) */
+    float rotate[2];      /* degrees about x, then about y (camera_rot) */
+    float fov_deg;        /* gluPerspective fovy (and setFOV) */
+    float znear, zfar;
+    float point_radius;   /* setradius: world-space sprite radius */
+    float max_point_size; /* largest sprite side in pixels, 1..256 (0: 64) */
+    float clear[4];       /* glClearColor */
+} cmag_render_view;
+typedef struct cmag_renderer cmag_renderer;
+/* the demo's defaults: camera (0.5, 0, -2.5), no rotation, 60 degrees,
+ * near 0.001, far 100, radius 0.0625, white background */
+void cmag_render_view_default(cmag_render_view* v);
+int cmag_renderer_create(uint32_t width, uint32_t height, cmag_renderer** out);
+int cmag_renderer_destroy(cmag_renderer* r);
+/* one frame of n particles (pos4: device float4 rows, color4: device float4
+ * RGBA or NULL for white), enqueued on stream */
+int cmag_renderer_display(cmag_renderer* r, const cmag_render_view* view, const float* pos4, const float* color4,
+                          uint32_t n, cmag_stream_t stream);
+/* the device image (width * height RGBA8, borrowed) and a synchronous copy of it to host memory */
+void* cmag_renderer_image(cmag_renderer* r);
+int cmag_renderer_read(cmag_renderer* r, uint8_t* rgba, cmag_stream_t stream);
+
 /* ---- adjoint of the Poiseuille step (no reference counterpart) -------------
  * Vector-Jacobian products of this library's forward passes with the step's
  * neighbour structure held fixed (it is piecewise constant); walls (w == 1)
