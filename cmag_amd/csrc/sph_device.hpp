@@ -725,7 +725,6 @@ struct NbEntry {
 // empties the queues.  push() has no room check: every site that pushes
 // leaves each queue below CAP before the next site may push.
 constexpr int NB_DRAIN = CMAG_NB_DRAIN;
-static_assert(NB_DRAIN >= 2, "the mask walk pushes two entries between drains");
 
 template <int CAP>
 struct NbQueue {
@@ -739,8 +738,6 @@ struct NbQueue {
         ++count;
     }
     __device__ __forceinline__ bool full() const { return count == CAP; }
-    // fewer than two free entries
-    __device__ __forceinline__ bool nearly_full() const { return count >= CAP - 1; }
     // f(e0, e1, second): the oldest entries two at a time; a lane with an odd
     // number left repeats its last entry with second = false (contributing
     // nothing), so both entries' rows can be loaded before either is used
@@ -949,9 +946,11 @@ struct ForceSum {
 // pair term gathers two rows instead of three and measures is not read.
 // With masks (staged blocks of the sub-cell walk), the in-range fluid
 // neighbours of a row whose main run holds at most MASK_BITS candidates come
-// from the density pass's mask instead of a walk, and the walls of its first two
-// cells are walked on their own; both keep the walk's order (pair terms and
-// wall terms accumulate apart), so the sums are the same bit for bit.
+// from the density pass's mask instead of a walk, and their pair terms are
+// evaluated at once, two at a time, after whatever earlier rows left in the
+// queue; the walls of the row's first two cells are walked on their own.  Pair
+// terms keep the walk's order and wall terms accumulate apart, so the sums are
+// the same bit for bit.
 template <int SYS, int DIM, int HASH, typename Cells, int CAP = NB_CAP>
 __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, float4* __restrict__ acc,
                                            const float4* __restrict__ measures,
@@ -1051,25 +1050,25 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
                         walk_run_sub(sortedPos, f0 + c0.w, fx, pi.x, pi.y, pi.z, 13, wall_only);
                         walk_run_sub(sortedPos, fx + cxw, f1, pi.x, pi.y, pi.z, 13, wall_only);
                     }
+                    // neighbours queued by earlier walked rows come first
+                    if (__ballot(nq.count != 0)) nq.drain(CAP, pairs);
                     // bit-reversed, candidate t of the run sits at bit 64 - 2K + t,
                     // i.e. candidate j at bit j - s0; the target's own bit is cleared
                     uint64_t m = __builtin_bitreverse64(((uint64_t)cur.y << 32) | cur.x);
                     const uint32_t s0 = s - (64u - ((e - s + 1u) & ~1u));
                     const uint32_t d = i - s0;
                     m &= ~(d < 64u ? (1ull << d) : 0ull);
-                    // two neighbours per iteration; the queue keeps room for both
+                    // the pair terms straight from the mask, two neighbours per
+                    // iteration in walk order (a missing second repeats the
+                    // first and adds nothing), without the queue
                     while (m != 0u) {
-                        if (__ballot(nq.nearly_full())) nq.drain(NB_DRAIN, pairs);
-                        nq.push(NbEntry::pack(s0 + (uint32_t)__builtin_ctzll(m), 13));
+                        const uint32_t j0 = s0 + (uint32_t)__builtin_ctzll(m);
                         m &= m - 1u;
-                        if (m != 0u) {
-                            nq.push(NbEntry::pack(s0 + (uint32_t)__builtin_ctzll(m), 13));
-                            m &= m - 1u;
-                        }
+                        const bool two = m != 0u;
+                        const uint32_t j1 = two ? s0 + (uint32_t)__builtin_ctzll(m) : j0;
+                        if (two) m &= m - 1u;
+                        pairs(NbEntry::pack(j0, 13), NbEntry::pack(j1, 13), two);
                     }
-                    // the last pushes may have filled a queue, and cand pushes
-                    // before it checks: leave every queue below CAP, as cand does
-                    if (__ballot(nq.full())) nq.drain(NB_DRAIN, pairs);
                 }
                 walk_run_sub(sortedPos, ws, we, pi.x, pi.y, pi.z, 13, cand);
             }
