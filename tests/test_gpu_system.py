@@ -248,19 +248,32 @@ def _mixed_box(p, n, seed, span=0.3, quarter_lattice=False):
     return pos
 
 
-@pytest.mark.parametrize("case", ["dam3d", "dam3d_linear", "quarter_boundaries", "ragged", "gated_1m"])
+@pytest.mark.parametrize("case", ["dam3d", "dam3d_linear", "quarter_boundaries", "ragged", "gated_1m", "dam2d"])
 def test_inrange_masks_change_nothing(device, case):
     """The force pass fed by the density pass's in-range masks (wall-free rows
     of staged blocks) pushes the same neighbours in the same order as the
     force pass that tests every candidate: the trajectories are the same bit
     for bit, eager and graph-replayed, with walls mixed into cells, particles
-    on quarter boundaries, ragged counts and the gated 1M dam break."""
-    if case == "gated_1m":
+    on quarter boundaries, ragged counts, the gated 1M dam break and a 2-D
+    dam break."""
+    if case == "dam2d":
+        # the reference's 2-D dam break (64 x 64 column) on cells of 2h
+        # (cellcount 1), so that its blocks stage their lookups and use masks
+        import ctypes as C
+        from oracle import oracle as O
+        fluid, bo, grid, r = (64, 64, 1), 1, (256, 128, 4), 1.0 / 128
+        p = S.params_dambreak(fluid, bo, grid, r)
+        p.cellcount = 1
+        for a in range(3):
+            p.cell_size[a] = 2.0 * p.smoothing_radius
+        L.call("cmag_params_finalize", C.byref(p))
+        pos, vel, steps = O.dambreak_init(p, O.dambreak_count(fluid, bo, grid)), None, 200
+    elif case == "gated_1m":
         sc = S.dambreak3d(dx=5e-3, hash_order=L.HASH_MORTON, morton_xbits=7)
         # long enough for rows whose mask walk fills a queue to be followed
         # by walked rows (the surge meets the floor and the far wall)
         p, pos, vel, steps = sc.params, sc.pos, sc.vel, 80
-    else:
+    if case not in ("dam2d", "gated_1m"):
         order, xbits = (L.HASH_LINEAR, 0) if case == "dam3d_linear" else (L.HASH_MORTON, 4)
         sc = S.dambreak3d(dx=0.02, hash_order=order, morton_xbits=xbits)
         p, vel, steps = sc.params, None, 6
