@@ -355,7 +355,7 @@ def main() -> int:
                     help="after the timed steps, run 1 GiB device copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); this build has no multi-GPU path",
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the library steps a system on one GPU (no multi-GPU path)",
               file=sys.stderr)
         return 2
 
