@@ -52,6 +52,30 @@ ALG_BYTES = {
 }
 
 
+# per particle and step for the batched channels (every stage, every particle):
+# hash 24, sort 16, reorder 72, density 32, then the fused force pass (sortedPos,
+# sortedVel, measures, index -> acc, or -> viscous + pressure force) and
+# integrate (pos, vel, acc, velLF [, presF] -> pos, vel, velLF)
+BATCH_BYTES = {
+    "poiseuille": 24 + 16 + 72 + 32 + (16 + 16 + 16 + 4 + 16) + (64 + 48),
+    "peristalsis": 24 + 16 + 72 + 32 + (16 + 16 + 16 + 4 + 32) + (80 + 48),
+}
+
+
+def batch_alg_bytes(scen) -> int:
+    """algorithmic bytes of one batched step: the per-particle bytes above and
+    each instance's cellStart fill (4 B a cell); the cell-range reads of the
+    occupied cells are not charged"""
+    from cmag_amd import _lib as L
+    from cmag_amd import scenarios as S
+
+    total = 0
+    for sc in scen:
+        kind = "peristalsis" if sc.params.system == L.SYS_PERISTALSIS else "poiseuille"
+        total += BATCH_BYTES[kind] * sc.pos.shape[0] + 4 * S.num_cells(sc.params)
+    return total
+
+
 def cell_bytes(stage: str, num_cells: int, occupied: int) -> int:
     """cell-array traffic per launch: the cellStart fill, the start/end writes of
     occupied cells in reorder, and one start/end read per occupied cell per pass"""
@@ -197,6 +221,8 @@ def bench_batch(steps: int = 1000, plain_steps: int = 20, tune_cells: bool = Tru
     batched = n_total * steps / wall
     plain = n_total * plain_steps / plain_wall
     plain_graph = n_total * plain_steps / plain_graph_wall
+    step_bytes = batch_alg_bytes(scen)
+    step_gbs = step_bytes / (wall / steps) / 1e9
     return {
         "workload": (f"256 2-D channels (128 Poiseuille nu~logU(1e-3,1e-1), 128 peristaltic A/H~U(0.1,0.5), "
                      f"wavelength~U(0.5,2)W), {n_total:,} particles, counts log-uniform 1k..64k"),
@@ -216,6 +242,9 @@ def bench_batch(steps: int = 1000, plain_steps: int = 20, tune_cells: bool = Tru
         "plain_loop_graph_particle_steps_per_s": plain_graph,
         "speedup_vs_plain_loop": batched / plain,
         "speedup_vs_plain_loop_graphs": batched / plain_graph,
+        "roofline": {"bound": "hbm", "kernel": "whole step", "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4), "alg_bytes_per_step": step_bytes,
+                     "traffic": None},
     }
 
 

@@ -423,7 +423,14 @@ def test_channel_2000_steps_matches_series_and_fp64_oracle(device, oracle):
     yc = (np.arange(64) + 0.5) / 64 * sc.meta["width"]
     k = S.lattice_laplacian_moment(sc.meta["h"], sc.meta["dx"], 2)
     want = S.poiseuille_series(yc, T, sc.meta["g"], sc.meta["width"], sc.meta["nu"] * k)
-    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 0.01
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    # against the series at the stated nu the Morris term's lattice moment
+    # (K = 0.974 at h = 1.3 dx) shows: 1.6 % in the FP32 oracle's run
+    raw = S.poiseuille_series(yc, T, sc.meta["g"], sc.meta["width"], sc.meta["nu"])
+    err_raw = np.linalg.norm(got - raw) / np.linalg.norm(raw)
+    print(f"channel L2 vs series: {err:.4f} at K*nu (K = {k:.4f}), {err_raw:.4f} at nu", flush=True)
+    assert err < 0.01
+    assert err_raw < 0.025
     sim = oracle.Simulation(sc.params, sc.pos, sc.vel, sc.vel, dtype=np.float64)
     sim.update(steps)
     ref = profile(sim.pos, sim.vel)
