@@ -157,6 +157,18 @@ struct Run {
     int code;
 };
 
+// The order in which the force pass takes the stencil rows (row r = (dz + 1)
+// * 3 + (dy + 1)): rows of opposite offsets in pairs, y - 1 with y + 1 in each
+// z plane, then z - 1 with z + 1, the centre row last.  A lane's in-range
+// neighbours are spread unevenly between opposite rows (a target near the top
+// of its cell has most of them above), so taking a pair's mask bits as one
+// sequence per lane evens the lanes' work out; the walk without masks takes
+// the rows in the same order, so the sums are the same.
+__device__ __forceinline__ int force_row(int dim, int k) {
+    // 3-D: 0 2 3 5 6 8 1 7 4; 2-D: 0 2 1
+    return dim == 3 ? (int)((0x471865320ull >> (4 * k)) & 15u) : (k == 0 ? 0 : k == 1 ? 2 : 1);
+}
+
 // Walk the stencil of gp in z, y, x order and call cand(j, qx, qy, qz, code)
 // for every candidate j in the reference's order.  Along x, cells with
 // consecutive keys form one run (linear order: the whole row; brick Z order:
@@ -168,8 +180,9 @@ struct Run {
 // PERIODIC = false compiles the image shifts out (DamBreak has none).
 //
 // walk_rows does the walk and hands each row's runs to row(a, b, qy, qz);
-// walk_stencil and walk_stencil_pf turn them into candidate calls.
-template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename R>
+// walk_stencil and walk_stencil_pf turn them into candidate calls.  With
+// FORCE_ORDER and one cell per stencil side, the rows come in force_row order.
+template <int DIM, int HASH, bool PERIODIC = true, bool FORCE_ORDER = false, typename Cells, typename R>
 __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi, const Cell& gp,
                                           const Cells& cells, R&& run_pair) {
     const int cc = P.cellcount;
@@ -191,74 +204,94 @@ __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi
             if (DIM == 3) kz0 = dilated_dec(kz0, Mz);
         }
     }
-    uint32_t kz = kz0;
-    for (int dz = zlo; dz <= zhi; ++dz) {
+    // one stencil row (dy, dz); ky, kz: its Morton key parts
+    auto row_body = [&](int dy, int dz, uint32_t ky, uint32_t kz) {
         const int nz = gp.z + dz;
         const int sz = perz ? image_shift(nz, P.grid_size[2]) : 0;
         const float pz = perz ? pi.z - (float)sz * P.world_size[2] : pi.z;
+        const int ny = gp.y + dy;
+        const int sy = pery ? image_shift(ny, P.grid_size[1]) : 0;
+        const float py = pery ? pi.y - (float)sy * P.world_size[1] : pi.y;
+        const int code_yz = 3 * (sy + 1) + 9 * (sz + 1);
+        const uint32_t row = (HASH == CMAG_HASH_LINEAR)
+                                 ? (((uint32_t)nz & gzm) * P.grid_size[1] + ((uint32_t)ny & gym)) * P.grid_size[0]
+                                 : (kz | ky);
+        Run a{0u, 0u, pi.x, 13}, b{0u, 0u, pi.x, 13};
+        bool have_a = false;
+        auto push = [&](uint32_t s, uint32_t e, float px, int code) {
+            const Run r{s, e - s, px, code};
+            if (!have_a) {
+                a = r;
+                have_a = true;
+            } else if (b.len == 0) {
+                b = r;
+            } else {  // a third run in one row (periodic seam with plain Z order, or walls)
+                run_pair(a, b, py, pz);
+                a = r;
+                b.len = 0;
+            }
+        };
+        // lo / hi: the run starts at the stencil's first cell / ends at its last
+        auto emit = [&](uint32_t k0, uint32_t k1, float px, int code, bool lo, bool hi) {
+            uint32_t s, e;
+            if constexpr (Cells::SUB) {
+                uint32_t ws, we;
+                cells.range_sub(k0, k1, lo, hi, sub, s, e, ws, we);
+                if (s < e) push(s, e, px, code);
+                if (ws < we) push(ws, we, px, code);
+            } else {
+                if (cells.range(k0, k1, s, e)) push(s, e, px, code);
+            }
+        };
+        uint32_t k0 = 0, k1 = 0;
+        float run_px = pi.x;
+        int run_sx = 0;
+        bool run_lo = true;
+        uint32_t kx = kx0;
+        for (int dx = -cc; dx <= cc; ++dx) {
+            const int nx = gp.x + dx;
+            const int sx = perx ? image_shift(nx, P.grid_size[0]) : 0;
+            const float px = perx ? pi.x - (float)sx * P.world_size[0] : pi.x;
+            const uint32_t key = (HASH == CMAG_HASH_LINEAR) ? row + ((uint32_t)nx & gxm) : (row | kx);
+            if (dx == -cc) {
+                k0 = k1 = key;
+                run_px = px;
+                run_sx = sx;
+            } else if (key == k1 + 1u && sx == run_sx) {
+                k1 = key;
+            } else {
+                emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, false);
+                k0 = k1 = key;
+                run_px = px;
+                run_sx = sx;
+                run_lo = false;
+            }
+            if (HASH == CMAG_HASH_MORTON) kx = dilated_inc(kx, Mx);
+        }
+        emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, true);
+        run_pair(a, b, py, pz);
+    };
+    if (FORCE_ORDER && cc == 1) {
+        // the force pass's row order (force_row), as its staged blocks take it;
+        // the Morton key parts are stepped from the stencil's first row each time
+        constexpr int NR = DIM == 3 ? 9 : 3;
+        for (int k = 0; k < NR; ++k) {
+            const int r = force_row(DIM, k);
+            const int iy = r % 3, iz = DIM == 3 ? r / 3 : 0;
+            uint32_t ky = ky0, kz = kz0;
+            if (HASH == CMAG_HASH_MORTON) {
+                for (int t = 0; t < iy; ++t) ky = dilated_inc(ky, My);
+                for (int t = 0; t < iz; ++t) kz = dilated_inc(kz, Mz);
+            }
+            row_body(iy - 1, DIM == 3 ? iz - 1 : 0, ky, kz);
+        }
+        return;
+    }
+    uint32_t kz = kz0;
+    for (int dz = zlo; dz <= zhi; ++dz) {
         uint32_t ky = ky0;
         for (int dy = -cc; dy <= cc; ++dy) {
-            const int ny = gp.y + dy;
-            const int sy = pery ? image_shift(ny, P.grid_size[1]) : 0;
-            const float py = pery ? pi.y - (float)sy * P.world_size[1] : pi.y;
-            const int code_yz = 3 * (sy + 1) + 9 * (sz + 1);
-            const uint32_t row = (HASH == CMAG_HASH_LINEAR)
-                                     ? (((uint32_t)nz & gzm) * P.grid_size[1] + ((uint32_t)ny & gym)) * P.grid_size[0]
-                                     : (kz | ky);
-            Run a{0u, 0u, pi.x, 13}, b{0u, 0u, pi.x, 13};
-            bool have_a = false;
-            auto push = [&](uint32_t s, uint32_t e, float px, int code) {
-                const Run r{s, e - s, px, code};
-                if (!have_a) {
-                    a = r;
-                    have_a = true;
-                } else if (b.len == 0) {
-                    b = r;
-                } else {  // a third run in one row (periodic seam with plain Z order, or walls)
-                    run_pair(a, b, py, pz);
-                    a = r;
-                    b.len = 0;
-                }
-            };
-            // lo / hi: the run starts at the stencil's first cell / ends at its last
-            auto emit = [&](uint32_t k0, uint32_t k1, float px, int code, bool lo, bool hi) {
-                uint32_t s, e;
-                if constexpr (Cells::SUB) {
-                    uint32_t ws, we;
-                    cells.range_sub(k0, k1, lo, hi, sub, s, e, ws, we);
-                    if (s < e) push(s, e, px, code);
-                    if (ws < we) push(ws, we, px, code);
-                } else {
-                    if (cells.range(k0, k1, s, e)) push(s, e, px, code);
-                }
-            };
-            uint32_t k0 = 0, k1 = 0;
-            float run_px = pi.x;
-            int run_sx = 0;
-            bool run_lo = true;
-            uint32_t kx = kx0;
-            for (int dx = -cc; dx <= cc; ++dx) {
-                const int nx = gp.x + dx;
-                const int sx = perx ? image_shift(nx, P.grid_size[0]) : 0;
-                const float px = perx ? pi.x - (float)sx * P.world_size[0] : pi.x;
-                const uint32_t key = (HASH == CMAG_HASH_LINEAR) ? row + ((uint32_t)nx & gxm) : (row | kx);
-                if (dx == -cc) {
-                    k0 = k1 = key;
-                    run_px = px;
-                    run_sx = sx;
-                } else if (key == k1 + 1u && sx == run_sx) {
-                    k1 = key;
-                } else {
-                    emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, false);
-                    k0 = k1 = key;
-                    run_px = px;
-                    run_sx = sx;
-                    run_lo = false;
-                }
-                if (HASH == CMAG_HASH_MORTON) kx = dilated_inc(kx, Mx);
-            }
-            emit(k0, k1, run_px, (run_sx + 1) + code_yz, run_lo, true);
-            run_pair(a, b, py, pz);
+            row_body(dy, dz, ky, kz);
             if (HASH == CMAG_HASH_MORTON) ky = dilated_inc(ky, My);
         }
         if (HASH == CMAG_HASH_MORTON) kz = dilated_inc(kz, Mz);
@@ -313,11 +346,12 @@ __device__ __forceinline__ void walk_run_sub(const float4* __restrict__ pos, uin
     } while (off < end);
 }
 
-template <int DIM, int HASH, bool PERIODIC = true, typename Cells, typename C>
+template <int DIM, int HASH, bool PERIODIC = true, bool FORCE_ORDER = false, typename Cells, typename C>
 __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const float4& pi, const Cell& gp,
                                                 const Cells& cells, const float4* __restrict__ pos, C&& cand) {
     if constexpr (Cells::SUB) {
-        walk_rows<DIM, HASH, PERIODIC>(P, pi, gp, cells, [&](const Run& a, const Run& b, float py, float pz) {
+        walk_rows<DIM, HASH, PERIODIC, FORCE_ORDER>(P, pi, gp, cells,
+                                                    [&](const Run& a, const Run& b, float py, float pz) {
             walk_run_sub(pos, a.s, a.s + a.len, a.px, py, pz, a.code, cand);
             walk_run_sub(pos, b.s, b.s + b.len, b.px, py, pz, b.code, cand);
         });
@@ -544,8 +578,8 @@ __device__ __forceinline__ void stage_rows(const LookupPlan& lp, const LookupTab
 
 // the candidates of target pi (cell x wx, wrapped) through the staged
 // lookups, as walk_stencil_pf hands them; with ROWS, the stencil rows that
-// stage_rows copied are read from LDS
-template <int DIM, bool ROWS = false, typename C>
+// stage_rows copied are read from LDS; with FORCE_ORDER, rows in force_row order
+template <int DIM, bool ROWS = false, bool FORCE_ORDER = false, typename C>
 __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& pi, const Cell& gp,
                                             const LookupPlan& lp, const LookupTables& lt,
                                             const float4* __restrict__ pos, C&& cand, const RowStage* rs = nullptr,
@@ -563,7 +597,8 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
             cand(j + 1u, p1, j + 1u < e, pi.x, pi.y, pi.z, 13);
         }
     };
-    for (int r = 0; r < R; ++r) {
+    for (int k = 0; k < R; ++k) {
+        const int r = FORCE_ORDER ? force_row(DIM, k) : k;
         const uint4 c0 = v.Pr(r, 0);
         const uint32_t s = v.F(r, 0) + SubCells::below(c0, q);
         const uint32_t f1 = v.F(r, 2);
@@ -1026,56 +1061,96 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
             constexpr int R = DIM == 3 ? 9 : 3;
             const int q = x_quarter(P, pi.x);
             const LookupView v = lookup_view<DIM>(P, *lp, *lt, gp);
-            // each row's mask is loaded one row ahead of its use
-            uint2 mw = masks[i];
-            for (int r = 0; r < R; ++r) {
-                const uint2 cur = mw;
-                if (r + 1 < R) mw = masks[(size_t)(r + 1) * stride + i];
+            // the main run [s, e) of row r: cells x - 1 (from quarter q) to
+            // x + 1 (through quarter q)
+            auto main_run = [&](int r, uint32_t& s, uint32_t& e) {
+                s = v.F(r, 0) + SubCells::below(v.Pr(r, 0), q);
+                e = v.F(r, 2) + SubCells::through(v.Pr(r, 2), q);
+            };
+            // the walls of a main run (those of its first two cells, in run
+            // order) when the run itself is not walked: the mask holds fluid only
+            auto run_walls = [&](int r) {
                 const uint4 c0 = v.Pr(r, 0);
-                const uint32_t f0 = v.F(r, 0);
-                const uint32_t s = f0 + SubCells::below(c0, q);
-                const uint32_t f1 = v.F(r, 2);
-                const uint4 c1 = v.Pr(r, 2);
-                const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
-                // walls of the run's first two cells (the last cell's are [ws, we))
-                const uint32_t cxw = v.Pr(r, 1).w;
-                const uint32_t walls = (f1 - f0) - c0.w - cxw;
-                if (e - s > (uint32_t)MASK_BITS) {
+                const uint32_t f0 = v.F(r, 0), fx = v.F(r, 1), f1 = v.F(r, 2), cxw = v.Pr(r, 1).w;
+                if ((f1 - f0) - c0.w - cxw != 0u) {
+                    walk_run_sub(sortedPos, f0 + c0.w, fx, pi.x, pi.y, pi.z, 13, wall_only);
+                    walk_run_sub(sortedPos, fx + cxw, f1, pi.x, pi.y, pi.z, 13, wall_only);
+                }
+            };
+            // the last cell's walls [ws, we)
+            auto last_walls = [&](int r) {
+                walk_run_sub(sortedPos, v.F(r, 2) + v.Pr(r, 2).w, v.F(r, 3), pi.x, pi.y, pi.z, 13, cand);
+            };
+            // row r's mask, bit-reversed: candidate t of the run sits at bit
+            // 64 - 2K + t, i.e. candidate j at bit j - s0; the target's own bit
+            // is cleared
+            auto row_mask = [&](const uint2 w, uint32_t s, uint32_t e, uint32_t& s0) {
+                uint64_t m = __builtin_bitreverse64(((uint64_t)w.y << 32) | w.x);
+                s0 = s - (64u - ((e - s + 1u) & ~1u));
+                const uint32_t d = i - s0;
+                return m & ~(d < 64u ? (1ull << d) : 0ull);
+            };
+            // pair terms straight from masks, two neighbours per iteration in
+            // walk order (a missing second repeats the first and adds nothing),
+            // without the queue: first mA's bits, then mB's
+            auto mask_pairs = [&](uint64_t mA, uint32_t sA0, uint64_t mB, uint32_t sB0) {
+                while ((mA | mB) != 0u) {
+                    if (mA == 0u) {
+                        mA = mB;
+                        sA0 = sB0;
+                        mB = 0u;
+                    }
+                    const uint32_t j0 = sA0 + (uint32_t)__builtin_ctzll(mA);
+                    mA &= mA - 1u;
+                    const bool two = mA != 0u;
+                    const uint32_t j1 = two ? sA0 + (uint32_t)__builtin_ctzll(mA) : j0;
+                    if (two) mA &= mA - 1u;
+                    pairs(NbEntry::pack(j0, 13), NbEntry::pack(j1, 13), two);
+                }
+            };
+            // rows in force_row order, two at a time (the centre row alone)
+            for (int k = 0; k < R; k += 2) {
+                const int ra = force_row(DIM, k), rb = k + 1 < R ? force_row(DIM, k + 1) : -1;
+                // both masks are loaded ahead of the walls (a walked row's is not used)
+                const uint2 wa = masks[(size_t)ra * stride + i];
+                const uint2 wb = rb >= 0 ? masks[(size_t)rb * stride + i] : make_uint2(0u, 0u);
+                uint32_t s, e, sA0 = 0u, sB0 = 0u;
+                uint64_t mA = 0u, mB = 0u;
+                main_run(ra, s, e);
+                const bool walk_a = e - s > (uint32_t)MASK_BITS;
+                if (walk_a) {
                     walk_run_sub(sortedPos, s, e, pi.x, pi.y, pi.z, 13, cand);
                 } else {
-                    if (walls != 0u) {
-                        // the walls of cells x - 1 and x in run order; the
-                        // mask holds the fluid only
-                        const uint32_t fx = v.F(r, 1);
-                        walk_run_sub(sortedPos, f0 + c0.w, fx, pi.x, pi.y, pi.z, 13, wall_only);
-                        walk_run_sub(sortedPos, fx + cxw, f1, pi.x, pi.y, pi.z, 13, wall_only);
+                    run_walls(ra);
+                    mA = row_mask(wa, s, e, sA0);
+                }
+                last_walls(ra);
+                if (rb >= 0) {
+                    main_run(rb, s, e);
+                    if (e - s > (uint32_t)MASK_BITS) {
+                        // row a's pair terms come before the ones row b queues
+                        if (!walk_a) {
+                            if (nq.count != 0) nq.drain(CAP, pairs);
+                            mask_pairs(mA, sA0, 0u, 0u);
+                            mA = 0u;
+                        }
+                        walk_run_sub(sortedPos, s, e, pi.x, pi.y, pi.z, 13, cand);
+                    } else {
+                        run_walls(rb);
+                        mB = row_mask(wb, s, e, sB0);
                     }
+                    last_walls(rb);
+                }
+                if (__ballot((mA | mB) != 0u)) {
                     // neighbours queued by earlier walked rows come first
                     if (__ballot(nq.count != 0)) nq.drain(CAP, pairs);
-                    // bit-reversed, candidate t of the run sits at bit 64 - 2K + t,
-                    // i.e. candidate j at bit j - s0; the target's own bit is cleared
-                    uint64_t m = __builtin_bitreverse64(((uint64_t)cur.y << 32) | cur.x);
-                    const uint32_t s0 = s - (64u - ((e - s + 1u) & ~1u));
-                    const uint32_t d = i - s0;
-                    m &= ~(d < 64u ? (1ull << d) : 0ull);
-                    // the pair terms straight from the mask, two neighbours per
-                    // iteration in walk order (a missing second repeats the
-                    // first and adds nothing), without the queue
-                    while (m != 0u) {
-                        const uint32_t j0 = s0 + (uint32_t)__builtin_ctzll(m);
-                        m &= m - 1u;
-                        const bool two = m != 0u;
-                        const uint32_t j1 = two ? s0 + (uint32_t)__builtin_ctzll(m) : j0;
-                        if (two) m &= m - 1u;
-                        pairs(NbEntry::pack(j0, 13), NbEntry::pack(j1, 13), two);
-                    }
+                    mask_pairs(mA, sA0, mB, sB0);
                 }
-                walk_run_sub(sortedPos, ws, we, pi.x, pi.y, pi.z, 13, cand);
             }
         } else if (lp) {
-            walk_lookup<DIM>(P, pi, gp, *lp, *lt, sortedPos, cand);
+            walk_lookup<DIM, false, true>(P, pi, gp, *lp, *lt, sortedPos, cand);
         } else {
-            walk_stencil_pf<DIM, HASH, false>(P, pi, gp, cells, sortedPos, cand);
+            walk_stencil_pf<DIM, HASH, false, true>(P, pi, gp, cells, sortedPos, cand);
         }
     } else {
         walk_stencil_pf<DIM, HASH, SYS != CMAG_SYS_DAMBREAK>(P, pi, gp, cells, sortedPos, cand);
