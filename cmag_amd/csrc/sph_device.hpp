@@ -158,15 +158,16 @@ struct Run {
 };
 
 // The order in which the force pass takes the stencil rows (row r = (dz + 1)
-// * 3 + (dy + 1)): rows of opposite offsets in pairs, y - 1 with y + 1 in each
-// z plane, then z - 1 with z + 1, the centre row last.  A lane's in-range
-// neighbours are spread unevenly between opposite rows (a target near the top
-// of its cell has most of them above), so taking a pair's mask bits as one
-// sequence per lane evens the lanes' work out; the walk without masks takes
-// the rows in the same order, so the sums are the same.
+// * 3 + (dy + 1)): rows of opposite offsets in pairs -- the diagonal corners
+// (y - 1, z - 1) with (y + 1, z + 1) and (y + 1, z - 1) with (y - 1, z + 1),
+// then y - 1 with y + 1 and z - 1 with z + 1 -- and the centre row last.  A
+// lane's in-range neighbours are spread unevenly between opposite rows (a
+// target near the top of its cell has most of them above), so taking a pair's
+// mask bits as one sequence per lane evens the lanes' work out; the walks
+// without masks take the rows in the same order, so the sums are the same.
 __device__ __forceinline__ int force_row(int dim, int k) {
-    // 3-D: 0 2 3 5 6 8 1 7 4; 2-D: 0 2 1
-    return dim == 3 ? (int)((0x471865320ull >> (4 * k)) & 15u) : (k == 0 ? 0 : k == 1 ? 2 : 1);
+    // 3-D: 0 8 2 6 3 5 1 7 4; 2-D: 0 2 1
+    return dim == 3 ? (int)((0x471536280ull >> (4 * k)) & 15u) : (k == 0 ? 0 : k == 1 ? 2 : 1);
 }
 
 // Walk the stencil of gp in z, y, x order and call cand(j, qx, qy, qz, code)
