@@ -1095,17 +1095,22 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
             // walk order (a missing second repeats the first and adds nothing),
             // without the queue: first mA's bits, then mB's
             auto mask_pairs = [&](uint64_t mA, uint32_t sA0, uint64_t mB, uint32_t sB0) {
-                while ((mA | mB) != 0u) {
+                // row b's bits follow once row a's are used up
+                auto next = [&]() {
                     if (mA == 0u) {
                         mA = mB;
                         sA0 = sB0;
                         mB = 0u;
                     }
+                };
+                next();
+                while (mA != 0u) {
                     const uint32_t j0 = sA0 + (uint32_t)__builtin_ctzll(mA);
                     mA &= mA - 1u;
                     const bool two = mA != 0u;
                     const uint32_t j1 = two ? sA0 + (uint32_t)__builtin_ctzll(mA) : j0;
                     if (two) mA &= mA - 1u;
+                    next();
                     pairs(NbEntry::pack(j0, 13), NbEntry::pack(j1, 13), two);
                 }
             };
