@@ -693,23 +693,17 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                 m = mask_shift_in(m, ra, rb, r2max);
             }
         } else {
-            // as walk_run_sub: two rows per iteration, each load one candidate ahead
+            // two rows per iteration over a byte offset
             const char* base = reinterpret_cast<const char*>(pos);
             auto ld = [&](uint32_t off) { return *reinterpret_cast<const float4*>(base + off); };
             const uint32_t end = e << 4, end1 = end - 16u;
-            uint32_t off = s << 4;
-            if (off < end) {
-                float4 p0 = ld(off);
-                do {
-                    const float4 p1 = ld(off + 16u);
-                    const float ra = d2(p0);
-                    term(ra);
-                    p0 = ld(off + 32u);
-                    const float rb = off < end1 ? d2(p1) : 1e30f;
-                    term(rb);
-                    m = mask_shift_in(m, ra, rb, r2max);
-                    off += 32u;
-                } while (off < end);
+            for (uint32_t off = s << 4; off < end; off += 32u) {
+                const float4 p0 = ld(off), p1 = ld(off + 16u);
+                const float ra = d2(p0);
+                const float rb = off < end1 ? d2(p1) : 1e30f;
+                term(ra);
+                term(rb);
+                m = mask_shift_in(m, ra, rb, r2max);
             }
         }
         // the walls of the run's first two cells are walked by the force pass
