@@ -59,12 +59,13 @@ __device__ __forceinline__ Cell grid_pos(const cmag_params& P, float x, float y,
     return c;
 }
 
-// quarter (0..3) of a position's cell along x, from the quotient that
+// eighth (0..7) of a position's cell along x, from the quotient that
 // grid_pos floors (the DIV_CELL rule); the sub-cell walk order buckets by it
-__device__ __forceinline__ int x_quarter(const cmag_params& P, float x) {
+constexpr int X_SUBS = 8;
+__device__ __forceinline__ int x_sub(const cmag_params& P, float x) {
 #pragma clang fp contract(off)
     const float u = (x - P.world_origin[0]) / P.cell_size[0];
-    return min(3, (int)((u - floorf(u)) * 4.0f));
+    return min(X_SUBS - 1, (int)((u - floorf(u)) * (float)X_SUBS));
 }
 
 // software bit deposit of the low bits of v into the set bits of mask

@@ -64,7 +64,7 @@ int launch_calc_hash(const cmag_params& P, const float* pos4, uint32_t* hash, ui
 // the previous cell; the last element ends its cell.  Gathers pos and the
 // leapfrog velocity into sorted order.
 // BUCKET: also the sub-cell bucket of every sorted row (DamBreak: fluid
-// quarter 0..3 of the cell along x, walls 4), one byte per row
+// eighth 0..7 of the cell along x, walls 8), one byte per row
 template <bool BUCKET>
 __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cellStart,
                                                        uint32_t* __restrict__ cellEnd,
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void reorder_kernel(uint32_t* __restrict__ cel
         const float4 p = oldPos[k];
         sortedPos[i] = p;
         sortedVel[i] = oldVel[k];
-        if (BUCKET) bucket[i] = (uint8_t)(p.w != CMAG_DB_FLUID ? 4 : x_quarter(P, p.x));
+        if (BUCKET) bucket[i] = (uint8_t)(p.w != CMAG_DB_FLUID ? X_SUBS : x_sub(P, p.x));
     }
 }
 

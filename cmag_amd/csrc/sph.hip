@@ -147,13 +147,13 @@ int launch_force(const cmag_params& P, float* acc, const float* measures, const 
 }
 
 // ---- sub-cell walk order (DamBreak) ---------------------------------------------
-// The bucket of every sorted row (a byte: fluid quarter 0..3 of the cell
-// along x, walls 4) comes from the reorder.  subcell_move_kernel: row i of
+// The bucket of every sorted row (a byte: fluid eighth 0..7 of the cell
+// along x, walls 8) comes from the reorder.  subcell_move_kernel: row i of
 // the reference order (cell c = hash[i]) moves to row first[c] + below of the
 // sub-cell order, below = the rows of the cell in a lower bucket or in the
 // same bucket and earlier in reference order; the cell's first row also
-// writes the running bucket counts pre[c] (cells without particles keep the
-// zeros they were cleared to).  O(cell size) work per row (~18 at the
+// writes the running bucket counts pre[c] (eight 16-bit counts; cells without
+// particles keep the zeros they were cleared to).  O(cell size) work per row (~18 at the
 // measured sizes) over 16-byte loads of the buckets.
 __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __restrict__ hash,
                                                            const uint32_t* __restrict__ index,
@@ -169,23 +169,25 @@ __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __res
     const uint32_t c = hash[i];
     const uint32_t a = first[c], b = min(first[c + 1], n);
     const uint32_t bi = bucket[i];
-    uint32_t below = 0, c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    uint32_t below = 0;
+    uint32_t cnt[X_SUBS] = {};  // fluid rows of eighths <= t
     // the cell's bucket bytes, 16 per load from a 16-byte boundary (the array
     // is padded so a load never reads past its end)
     for (uint32_t k16 = a & ~15u; k16 < b; k16 += 16u) {
         const uint4 v = *reinterpret_cast<const uint4*>(bucket + k16);
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        // byte t of acc counts this chunk's rows of eighths <= t (at most 16)
+        uint64_t acc = 0;
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
             const uint32_t k = k16 + (uint32_t)j;
             if (k < a || k >= b) continue;
             const uint32_t bk = (w[j >> 2] >> (8 * (j & 3))) & 0xffu;
             below += (bk < bi || (bk == bi && k < i)) ? 1u : 0u;
-            c0 += bk <= 0u ? 1u : 0u;
-            c1 += bk <= 1u ? 1u : 0u;
-            c2 += bk <= 2u ? 1u : 0u;
-            c3 += bk <= 3u ? 1u : 0u;
+            acc += bk < (uint32_t)X_SUBS ? 0x0101010101010101ull << (8u * bk) : 0ull;
         }
+#pragma unroll
+        for (int t = 0; t < X_SUBS; ++t) cnt[t] += (uint32_t)(acc >> (8 * t)) & 0xffu;
     }
     const uint32_t w = a + below;
     if (w >= n) return;  // inconsistent cell arrays only
@@ -198,7 +200,14 @@ __global__ __launch_bounds__(256) void subcell_move_kernel(const uint32_t* __res
     velW[w] = sortedVel[i];
     indexW[w] = index[i];
     w2r[w] = i;
-    if (i == a) pre[c] = make_uint4(c0, c1, c2, c3);
+    // 16-bit counts, capped: a cell of more than 65,535 rows (no physical
+    // state has one) keeps its runs inside the cell
+    if (i == a) {
+        uint32_t h[X_SUBS];
+#pragma unroll
+        for (int t = 0; t < X_SUBS; ++t) h[t] = min(cnt[t], 0xffffu);
+        pre[c] = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16);
+    }
 }
 
 int launch_subcell_build(const cmag_params& P, const uint32_t* hash, const uint32_t* index, const float* sortedPos,
@@ -277,13 +286,13 @@ int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2
 // global walk).  CAP: neighbour queue depth; with the in-range masks only
 // walked rows and blocks without staged lookups queue neighbours, and a
 // shallower queue leaves more LDS to the CU (16M dam break: 4.18 vs 4.29 ms
-// at depth 8 vs 16)
+// at depth 8 vs 16).  Registers are held to 72 a lane so that 7 waves share a
+// SIMD (16M dam break: 3.61 ms, against 3.75 ms at 76 registers and 6 waves);
+// at depth 16 the LDS allows fewer blocks anyway.
 template <int DIM, int HASH, int CAP>
-__global__ __launch_bounds__(256) void force_sub_kernel(const cmag_params P, float4* __restrict__ acc,
-                                                         const float4* __restrict__ posW,
-                                                         const float4* __restrict__ velW,
-                                                         const uint32_t* __restrict__ indexW, const SubCells cells,
-                                                         uint32_t n, const uint2* __restrict__ masks) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void force_sub_kernel(
+    const cmag_params P, float4* __restrict__ acc, const float4* __restrict__ posW, const float4* __restrict__ velW,
+    const uint32_t* __restrict__ indexW, const SubCells cells, uint32_t n, const uint2* __restrict__ masks) {
     __shared__ uint32_t nbl[256 / 64][CAP][64];
     __shared__ LookupPlan lp;
     __shared__ LookupTables lt;

@@ -102,45 +102,47 @@ struct FirstCells {
 
 // ---- sub-cell walk order (DamBreak) ---------------------------------------------
 // Inside each cell the particles can be kept in a second, internal order: by
-// bucket, then in reference order, with buckets 0..3 the fluid particles of
-// each quarter of the cell along x and bucket 4 the wall particles.  A
-// target in quarter q of its cell needs, of the stencil's first cell along
-// x, only the fluid of quarters >= q (the rest lies more than cellcount cells
-// away, beyond 2h), and of the last cell only the quarters <= q.  Wall
-// particles sit in their own bucket and are always walked: DamBreak's
-// Lennard-Jones term acts on every wall particle of the stencil.
-// x_quarter (common.hpp) is the target's / particle's quarter.
+// bucket, then in reference order, with buckets 0..7 the fluid particles of
+// each eighth of the cell along x and bucket 8 the wall particles.  A target
+// in eighth q of its cell needs, of the stencil's first cell along x, only the
+// fluid of eighths >= q (the rest lies more than cellcount cells away, beyond
+// 2h), and of the last cell only the eighths <= q.  Wall particles sit in
+// their own bucket and are always walked: DamBreak's Lennard-Jones term acts
+// on every wall particle of the stencil.  x_sub (common.hpp) is the target's /
+// particle's eighth.
 
 // SubCells: cell offsets first[] as FirstCells plus, per cell, the running
-// counts pre[c] = (#bucket 0, #buckets <= 1, #buckets <= 2, #fluid); cells
-// without particles hold zeros
+// counts pre[c]: eight 16-bit counts, #buckets <= t for t = 0..7 (t = 7: the
+// cell's fluid), two to a word, low half first; cells without particles hold
+// zeros
 struct SubCells {
     static constexpr bool SUB = true;
     const uint32_t* __restrict__ first;
     const uint4* __restrict__ pre;
-    // one of four values by q (0..3) with selects rather than branches (the
-    // quarter differs between the lanes of a wave)
-    __device__ __forceinline__ static uint32_t pick4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, int q) {
-        const uint32_t lo = (q & 1) ? b : a;
-        const uint32_t hi = (q & 1) ? d : c;
-        return (q & 2) ? hi : lo;
+    // count t (0..7) of a cell's entry, read as one 16-bit value (the eighth
+    // differs between the lanes of a wave)
+    __device__ __forceinline__ static uint32_t count(const uint4* c, int t) {
+        return reinterpret_cast<const uint16_t*>(c)[t];
     }
-    // the fluid of quarters <= q, and of quarters < q
-    __device__ __forceinline__ static uint32_t through(const uint4& c, int q) { return pick4(c.x, c.y, c.z, c.w, q); }
-    __device__ __forceinline__ static uint32_t below(const uint4& c, int q) { return pick4(0u, c.x, c.y, c.z, q); }
-    // the run k0..k1 trimmed for a target in quarter q: lo (k0 is the
-    // stencil's first cell along x) drops k0's fluid quarters < q, hi (k1 is
-    // the last) ends at k1's quarter q and returns k1's walls as [ws, we)
+    // the fluid of eighths <= q, of eighths < q, and all of it
+    __device__ __forceinline__ static uint32_t through(const uint4* c, int q) { return count(c, q); }
+    __device__ __forceinline__ static uint32_t below(const uint4* c, int q) {
+        const uint32_t v = count(c, q > 0 ? q - 1 : 0);
+        return q > 0 ? v : 0u;
+    }
+    __device__ __forceinline__ static uint32_t fluid(const uint4* c) { return count(c, X_SUBS - 1); }
+    // the run k0..k1 trimmed for a target in eighth q: lo (k0 is the
+    // stencil's first cell along x) drops k0's fluid eighths < q, hi (k1 is
+    // the last) ends at k1's eighth q and returns k1's walls as [ws, we)
     __device__ __forceinline__ void range_sub(uint32_t k0, uint32_t k1, bool lo, bool hi, int q, uint32_t& s,
                                               uint32_t& e, uint32_t& ws, uint32_t& we) const {
         s = first[k0];
-        if (lo) s += below(pre[k0], q);
+        if (lo) s += below(pre + k0, q);
         ws = we = 0;
         if (hi) {
             const uint32_t f1 = first[k1];
-            const uint4 c1 = pre[k1];
-            e = f1 + through(c1, q);
-            ws = f1 + c1.w;
+            e = f1 + through(pre + k1, q);
+            ws = f1 + fluid(pre + k1);
             we = first[k1 + 1];
         } else {
             e = first[k1 + 1];
@@ -188,7 +190,7 @@ __device__ __forceinline__ void walk_rows(const cmag_params& P, const float4& pi
                                           const Cells& cells, R&& run_pair) {
     const int cc = P.cellcount;
     const int zlo = DIM == 2 ? 0 : -cc, zhi = DIM == 2 ? 0 : cc;
-    const int sub = Cells::SUB ? x_quarter(P, pi.x) : 0;
+    const int sub = Cells::SUB ? x_sub(P, pi.x) : 0;
     const uint32_t gxm = P.grid_size[0] - 1u, gym = P.grid_size[1] - 1u, gzm = P.grid_size[2] - 1u;
     const uint32_t Mx = P.morton_mask[0], My = P.morton_mask[1], Mz = P.morton_mask[2];
     const bool perx = PERIODIC && (P.periodic_mask & 1), pery = PERIODIC && (P.periodic_mask & 2),
@@ -387,7 +389,7 @@ __device__ __forceinline__ void walk_stencil_pf(const cmag_params& P, const floa
 // end of one and the start of the next in key order), with each row's stencil
 // x-range [cx0 - 1, cx1 + 1] inside the grid, on consecutive keys and at most
 // LK_W cells wide (both rows together when there are two), copies the compact offsets
-// and quarter counts of those cells in the 9 (2-D: 3) stencil rows of each of
+// and eighth counts of those cells in the 9 (2-D: 3) stencil rows of each of
 // its rows into LDS once; its targets then find their runs there instead of
 // with five loads per stencil row.  The runs and their order are the global
 // walk's, so the sums are the same bit for bit.
@@ -418,7 +420,7 @@ struct LookupView {
     const uint4* pre;
     uint32_t fs, ps;
     __device__ __forceinline__ uint32_t F(int r, uint32_t c) const { return first[(uint32_t)r * fs + c]; }
-    __device__ __forceinline__ uint4 Pr(int r, uint32_t c) const { return pre[(uint32_t)r * ps + c]; }
+    __device__ __forceinline__ const uint4* Pr(int r, uint32_t c) const { return pre + (uint32_t)r * ps + c; }
 };
 
 template <int DIM>
@@ -586,7 +588,7 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
                                             const float4* __restrict__ pos, C&& cand, const RowStage* rs = nullptr,
                                             const RowLds* lds = nullptr) {
     constexpr int R = DIM == 3 ? 9 : 3;
-    const int q = x_quarter(P, pi.x);
+    const int q = x_sub(P, pi.x);
     const LookupView v = lookup_view<DIM>(P, lp, lt, gp);
     auto run_lds = [&](uint32_t s, uint32_t e, int32_t off) {
         for (uint32_t j = s; j < e; j += 2u) {
@@ -600,11 +602,10 @@ __device__ __forceinline__ void walk_lookup(const cmag_params& P, const float4& 
     };
     for (int k = 0; k < R; ++k) {
         const int r = FORCE_ORDER ? force_row(DIM, k) : k;
-        const uint4 c0 = v.Pr(r, 0);
-        const uint32_t s = v.F(r, 0) + SubCells::below(c0, q);
+        const uint32_t s = v.F(r, 0) + SubCells::below(v.Pr(r, 0), q);
         const uint32_t f1 = v.F(r, 2);
-        const uint4 c1 = v.Pr(r, 2);
-        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
+        const uint4* c1 = v.Pr(r, 2);
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + SubCells::fluid(c1), we = v.F(r, 3);
         if (ROWS && rs->ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS) {
             run_lds(s, e, rs->off[r - ROW_STAGE_FIRST]);
             run_lds(ws, we, rs->off[r - ROW_STAGE_FIRST]);
@@ -656,7 +657,7 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
                                                    uint32_t i, uint32_t stride) {
     constexpr int R = DIM == 3 ? 9 : 3;
     const Cell gp = grid_pos(P, pi.x, pi.y, pi.z);
-    const int q = x_quarter(P, pi.x);
+    const int q = x_sub(P, pi.x);
     const LookupView v = lookup_view<DIM>(P, lp, lt, gp);
     const float half_inv_h = -0.5f / P.smoothing_radius;
     const float r2max = 4.0f * P.smoothing_radius * P.smoothing_radius;
@@ -673,12 +674,12 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
         term(valid ? d2(pj) : 1e30f);
     };
     for (int r = 0; r < R; ++r) {
-        const uint4 c0 = v.Pr(r, 0);
+        const uint4* c0 = v.Pr(r, 0);
         const uint32_t f0 = v.F(r, 0);
         const uint32_t s = f0 + SubCells::below(c0, q);
         const uint32_t f1 = v.F(r, 2);
-        const uint4 c1 = v.Pr(r, 2);
-        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + c1.w, we = v.F(r, 3);
+        const uint4* c1 = v.Pr(r, 2);
+        const uint32_t e = f1 + SubCells::through(c1, q), ws = f1 + SubCells::fluid(c1), we = v.F(r, 3);
         uint64_t m = 0;
         const bool in_lds = DIM == 3 && rs.ok && r >= ROW_STAGE_FIRST && r < ROW_STAGE_FIRST + ROW_STAGE_ROWS;
         if (in_lds) {
@@ -708,10 +709,10 @@ __device__ __forceinline__ float density_sum_masks(const cmag_params& P, const f
         }
         // the walls of the run's first two cells are walked by the force pass
         // (every wall of the stencil acts): their bits are cleared
-        const uint32_t fx = v.F(r, 1), cxw = v.Pr(r, 1).w;
+        const uint32_t fx = v.F(r, 1), cxw = SubCells::fluid(v.Pr(r, 1));
         const uint32_t K2 = (e - s + 1u) & ~1u;
-        if ((f1 - f0) - c0.w - cxw != 0u && K2 <= (uint32_t)MASK_BITS)
-            m &= ~(mask_bits(K2, f0 + c0.w - s, fx - s) | mask_bits(K2, fx + cxw - s, f1 - s));
+        if ((f1 - f0) - SubCells::fluid(c0) - cxw != 0u && K2 <= (uint32_t)MASK_BITS)
+            m &= ~(mask_bits(K2, f0 + SubCells::fluid(c0) - s, fx - s) | mask_bits(K2, fx + cxw - s, f1 - s));
         if (fluid) masks[(size_t)r * stride + i] = make_uint2((uint32_t)m, (uint32_t)(m >> 32));
         if (in_lds) {
             const int32_t off = rs.off[r - ROW_STAGE_FIRST];
@@ -1054,10 +1055,10 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
     if constexpr (Cells::SUB) {
         if (lp && masks) {
             constexpr int R = DIM == 3 ? 9 : 3;
-            const int q = x_quarter(P, pi.x);
+            const int q = x_sub(P, pi.x);
             const LookupView v = lookup_view<DIM>(P, *lp, *lt, gp);
-            // the main run [s, e) of row r: cells x - 1 (from quarter q) to
-            // x + 1 (through quarter q)
+            // the main run [s, e) of row r: cells x - 1 (from eighth q) to
+            // x + 1 (through eighth q)
             auto main_run = [&](int r, uint32_t& s, uint32_t& e) {
                 s = v.F(r, 0) + SubCells::below(v.Pr(r, 0), q);
                 e = v.F(r, 2) + SubCells::through(v.Pr(r, 2), q);
@@ -1065,16 +1066,16 @@ __device__ __forceinline__ void force_body(const cmag_params& P, uint32_t i, flo
             // the walls of a main run (those of its first two cells, in run
             // order) when the run itself is not walked: the mask holds fluid only
             auto run_walls = [&](int r) {
-                const uint4 c0 = v.Pr(r, 0);
-                const uint32_t f0 = v.F(r, 0), fx = v.F(r, 1), f1 = v.F(r, 2), cxw = v.Pr(r, 1).w;
-                if ((f1 - f0) - c0.w - cxw != 0u) {
-                    walk_run_sub(sortedPos, f0 + c0.w, fx, pi.x, pi.y, pi.z, 13, wall_only);
+                const uint4* c0 = v.Pr(r, 0);
+                const uint32_t f0 = v.F(r, 0), fx = v.F(r, 1), f1 = v.F(r, 2), cxw = SubCells::fluid(v.Pr(r, 1));
+                if ((f1 - f0) - SubCells::fluid(c0) - cxw != 0u) {
+                    walk_run_sub(sortedPos, f0 + SubCells::fluid(c0), fx, pi.x, pi.y, pi.z, 13, wall_only);
                     walk_run_sub(sortedPos, fx + cxw, f1, pi.x, pi.y, pi.z, 13, wall_only);
                 }
             };
             // the last cell's walls [ws, we)
             auto last_walls = [&](int r) {
-                walk_run_sub(sortedPos, v.F(r, 2) + v.Pr(r, 2).w, v.F(r, 3), pi.x, pi.y, pi.z, 13, cand);
+                walk_run_sub(sortedPos, v.F(r, 2) + SubCells::fluid(v.Pr(r, 2)), v.F(r, 3), pi.x, pi.y, pi.z, 13, cand);
             };
             // row r's mask, bit-reversed: candidate t of the run sits at bit
             // 64 - 2K + t, i.e. candidate j at bit j - s0; the target's own bit

@@ -230,7 +230,7 @@ int cmag_system_set_stream(cmag_system* s, cmag_stream_t stream);
 /* Neighbour-walk order of a DamBreak system (no counterpart in the reference,
  * whose walk is the only order): with on != 0 the density and force passes
  * run over an internal copy of the sorted rows ordered inside each cell by
- * x-quarter (fluid) with the wall particles last, and skip the quarters of the
+ * x-eighth (fluid) with the wall particles last, and skip the eighths of the
  * stencil's outer cells that lie beyond 2h.  Every array a caller reads
  * (hash, index, cell ranges, sorted rows, measures in reference order, forces)
  * keeps its meaning; density and force sums run in another order, so they

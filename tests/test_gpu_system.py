@@ -150,10 +150,11 @@ def test_subcell_walk_matches_plain_walk_and_oracle(device, oracle, order, xbits
     assert rel_field(a.getArray(L.ARRAY_POSITION), sim.pos) <= 1e-6
 
 
-def test_subcell_walk_on_quarter_boundaries(device, oracle):
-    """particles exactly on the quarter boundaries of their cells (x on a
-    lattice of cell/4), fluid and wall mixed in every cell: the sub-cell walk
-    equals the reference-order walk to rounding and the oracle to 1e-5"""
+def test_subcell_walk_on_eighth_boundaries(device, oracle):
+    """particles exactly on the eighth boundaries of their cells (x on a
+    lattice of cell/8, the boundaries of the sub-cell walk's buckets), fluid and
+    wall mixed in every cell: the sub-cell walk equals the reference-order walk
+    to rounding and the oracle to 1e-5"""
     sc = S.dambreak3d(dx=0.02, hash_order=L.HASH_MORTON, morton_xbits=4)
     p = sc.params
     cs = float(p.cell_size[0])
@@ -161,7 +162,7 @@ def test_subcell_walk_on_quarter_boundaries(device, oracle):
     n = 6000
     x0 = float(p.world_origin[0]) + cs * np.floor((0.05 - float(p.world_origin[0])) / cs)
     pos = np.zeros((n, 4), np.float32)
-    pos[:, 0] = x0 + (cs / 4) * rng.integers(0, 40, n)
+    pos[:, 0] = x0 + (cs / 8) * rng.integers(0, 80, n)
     pos[:, 1] = rng.uniform(0.0, 0.3, n)
     pos[:, 2] = rng.uniform(0.0, 0.3, n)
     pos[:, 3] = np.where(rng.uniform(size=n) < 0.2, 1.0, 4.0)  # CMAG_DB_FIRST walls, fluid
@@ -236,25 +237,25 @@ def test_subcell_walk_independent_of_hash_order(device):
             assert np.array_equal(got, want)
 
 
-def _mixed_box(p, n, seed, span=0.3, quarter_lattice=False):
+def _mixed_box(p, n, seed, span=0.3, eighth_lattice=False):
     rng = np.random.default_rng(seed)
     pos = np.zeros((n, 4), np.float32)
     pos[:, :3] = rng.uniform(0.0, span, (n, 3))
-    if quarter_lattice:
+    if eighth_lattice:
         cs = float(p.cell_size[0])
         x0 = float(p.world_origin[0]) + cs * np.floor((0.05 - float(p.world_origin[0])) / cs)
-        pos[:, 0] = x0 + (cs / 4) * rng.integers(0, 40, n)
+        pos[:, 0] = x0 + (cs / 8) * rng.integers(0, 80, n)
     pos[:, 3] = np.where(rng.uniform(size=n) < 0.2, 1.0, 4.0)  # CMAG_DB_FIRST walls, fluid
     return pos
 
 
-@pytest.mark.parametrize("case", ["dam3d", "dam3d_linear", "quarter_boundaries", "ragged", "gated_1m", "dam2d"])
+@pytest.mark.parametrize("case", ["dam3d", "dam3d_linear", "eighth_boundaries", "ragged", "gated_1m", "dam2d"])
 def test_inrange_masks_change_nothing(device, case):
     """The force pass fed by the density pass's in-range masks (wall-free rows
     of staged blocks) pushes the same neighbours in the same order as the
     force pass that tests every candidate: the trajectories are the same bit
     for bit, eager and graph-replayed, with walls mixed into cells, particles
-    on quarter boundaries, ragged counts, the gated 1M dam break and a 2-D
+    on eighth boundaries, ragged counts, the gated 1M dam break and a 2-D
     dam break."""
     if case == "dam2d":
         # the reference's 2-D dam break (64 x 64 column) on cells of 2h
@@ -278,7 +279,7 @@ def test_inrange_masks_change_nothing(device, case):
         sc = S.dambreak3d(dx=0.02, hash_order=order, morton_xbits=xbits)
         p, vel, steps = sc.params, None, 6
         pos = {"dam3d": sc.pos, "dam3d_linear": sc.pos,
-               "quarter_boundaries": _mixed_box(p, 6000, 7, quarter_lattice=True),
+               "eighth_boundaries": _mixed_box(p, 6000, 7, eighth_lattice=True),
                "ragged": _mixed_box(p, 4099, 4099)}[case]
     print(f"{case}: {pos.shape[0]} particles", flush=True)
     a = ParamSystem.from_state(p, pos, vel)
