@@ -286,11 +286,11 @@ int launch_density_sub(const cmag_params& P, float* measures, const uint32_t* w2
 // global walk).  CAP: neighbour queue depth; with the in-range masks only
 // walked rows and blocks without staged lookups queue neighbours, and a
 // shallower queue leaves more LDS to the CU (16M dam break: 4.18 vs 4.29 ms
-// at depth 8 vs 16).  Registers are held to 72 a lane so that 7 waves share a
-// SIMD (16M dam break: 3.61 ms, against 3.75 ms at 76 registers and 6 waves);
-// at depth 16 the LDS allows fewer blocks anyway.
+// at depth 8 vs 16).  At depth 8 registers are held to 72 a lane so that 7
+// waves share a SIMD (16M dam break: 3.61 ms, against 3.75 ms at 76 registers
+// and 6 waves); at depth 16 the queues' LDS allows fewer waves anyway.
 template <int DIM, int HASH, int CAP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7, 8))) void force_sub_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CAP <= 8 ? 7 : 1, 8))) void force_sub_kernel(
     const cmag_params P, float4* __restrict__ acc, const float4* __restrict__ posW, const float4* __restrict__ velW,
     const uint32_t* __restrict__ indexW, const SubCells cells, uint32_t n, const uint2* __restrict__ masks) {
     __shared__ uint32_t nbl[256 / 64][CAP][64];
